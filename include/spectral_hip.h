@@ -1,0 +1,285 @@
+/*
+ * spectral_hip.h -- C ABI of the MI355X (gfx950) light-transport core.
+ *
+ * Drop-in for spectral-core's three light-transport entry points:
+ *   camera render  CpuTracer::new + Renderer::accumulate   (spectral-core/src/tracer.rs:24-41,115-141;
+ *                                                           spectral-core/src/lib.rs:30-34)
+ *   lighttrace     lighttrace::trace                       (spectral-core/src/lighttrace.rs:157-166)
+ *   volume         volume::render_volumetric_scene_point,  (spectral-core/src/volume.rs:99-168,
+ *                  render_volumetric_scene, render_volumetric   233-329, 22-85)
+ * and for the GPU mirror's constructors/methods (spectral-gpu/src/lib.rs:179-388,
+ * spectral-gpu/src/vol_photons.rs:275-779).
+ *
+ * Conventions
+ *  - Every entry point returns an int status: SPX_OK (0) or a negative code.
+ *    The message of the last failure is spx_last_error(ctx).  Nothing
+ *    unwinds across the boundary.
+ *  - Films are caller-owned DEVICE buffers of 3*w*h floats, interleaved XYZ,
+ *    row 0 = top (the reference's Vec<[f32;3]> layout, lib.rs:10).  Render
+ *    calls ADD into them (AccumBuffer::sum semantics): the caller clears a
+ *    film with spx_fill_f32 and divides by its sample count.
+ *  - `stream` is a hipStream_t (NULL = the null stream).  Calls are
+ *    asynchronous on that stream unless documented otherwise.
+ *  - One context per thread; concurrency comes from streams.
+ *  - Struct layouts are byte-compatible with spectral-gpu/src/upload.rs:9-32.
+ */
+#ifndef SPECTRAL_HIP_H
+#define SPECTRAL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SPX_OK 0
+#define SPX_EINVAL (-1)   /* invalid argument (null pointer, zero size, bad enum) */
+#define SPX_ENOMEM (-2)   /* device allocation failed */
+#define SPX_EDEVICE (-3)  /* HIP runtime error */
+#define SPX_ECAP (-4)     /* a fixed capacity was exceeded (scene size, debug count) */
+#define SPX_EBUSY (-5)    /* serving: request not admitted under the memory cap */
+
+/* ---- enums ------------------------------------------------------------- */
+#define SPX_PRIM_SPHERE 0u
+#define SPX_PRIM_SOLID 1u
+#define SPX_MAT_LAMBERTIAN 0u
+#define SPX_MAT_DIELECTRIC 1u
+#define SPX_GLASS_BK7 0u
+#define SPX_GLASS_SF11 1u
+#define SPX_GLASS_WATER 2u
+#define SPX_ILLUM_D65 0u
+#define SPX_ILLUM_A 1u
+#define SPX_ILLUM_E 2u /* equal energy: the reference's photon power of 1 */
+#define SPX_EST_POINT 0u
+#define SPX_EST_BEAM 1u
+#define SPX_EST_LEGACY 2u
+#define SPX_ENGINE_PLAIN 0u     /* one thread per path/photon, one launch */
+#define SPX_ENGINE_WAVEFRONT 1u /* per-bounce SoA queues + compaction */
+
+/* Scene capacity (the scene is staged in LDS by every kernel). */
+#define SPX_MAX_PRIMS 16u
+#define SPX_MAX_PLANES 64u
+#define SPX_MAX_MATERIALS 16u
+
+/* ---- PODs ---------------------------------------------------------------- */
+/* GpuPrimitive, upload.rs:9-16 */
+typedef struct {
+    float center[3];
+    float radius;
+    uint32_t plane_start;
+    uint32_t plane_count;
+    uint32_t kind;     /* SPX_PRIM_* */
+    uint32_t material; /* index into the material array */
+} spx_prim;
+
+/* GpuPlane, upload.rs:29-32: half-space {x : normal.x + d <= 0}, normal out. */
+typedef struct {
+    float normal[3];
+    float d;
+} spx_plane;
+
+/* GpuMaterial, upload.rs:20-25 */
+typedef struct {
+    float reflectance; /* Lambertian albedo */
+    uint32_t glass;    /* SPX_GLASS_* */
+    uint32_t kind;     /* SPX_MAT_* */
+    uint32_t pad;
+} spx_material;
+
+/* Scene::{background, horizon} (scene.rs:19-26, 44-50).  sun_* is an
+ * extension (a directional sun disk, zero radiance = absent): escaped rays
+ * with dot(dir, sun_dir) >= sun_cos add sun_radiance. */
+typedef struct {
+    float background;
+    float horizon;
+    uint32_t has_horizon;
+    float sun_radiance;
+    float sun_dir[3];
+    float sun_cos;
+} spx_env;
+
+/* Camera::view_basis (camera.rs:36-38). */
+typedef struct {
+    float origin[3];
+    float lower_left[3];
+    float horizontal[3];
+    float vertical[3];
+} spx_camera;
+
+/* lighttrace::Beam (lighttrace.rs:116-121). */
+typedef struct {
+    float corner[3];
+    float u[3];
+    float v[3];
+    float dir[3];
+} spx_beam;
+
+/* lighttrace::Screen geometry (lighttrace.rs:15-34). */
+typedef struct {
+    float corner[3];
+    float u[3];
+    float v[3];
+    uint32_t width;
+    uint32_t height;
+} spx_screen;
+
+/* Render options.  {illuminant, dispersion=1, max_depth=8, rr_depth=0}
+ * reproduces the reference. */
+typedef struct {
+    uint32_t illuminant; /* SPX_ILLUM_*; camera: background SPD; photons: beam SPD */
+    uint32_t dispersion; /* 1 = n(lambda); 0 = n(550 nm) (vol_photons.rs:77 `spectral`) */
+    uint32_t max_depth;  /* bounces (tracer.rs:39, lighttrace.rs:133, volume.rs:126): 1..8 */
+    uint32_t rr_depth;   /* camera: 0 = no roulette; k = Russian roulette from bounce k */
+} spx_opts;
+
+/* Volume weights (volume.rs:99-112; VolWeights, vol_photons.rs:256-267). */
+typedef struct {
+    float sigma_s;
+    float sigma_t;
+    float g;
+    float max_dist;
+} spx_vol_params;
+
+/* Per-photon ray states (DebugPhotonGpu, vol_photons.rs:87-93). */
+typedef struct {
+    uint32_t num_states; /* 1 + successful scatters */
+    float lambda;
+    uint32_t tir_count;  /* bounces whose refract() had no solution */
+    uint32_t pad;
+    float states[9][2][4]; /* [origin.xyz,_][dir.xyz,_] */
+} spx_photon_record;
+
+typedef struct spx_ctx spx_ctx;
+typedef struct spx_scene spx_scene;
+
+/* ---- context --------------------------------------------------------------- */
+/* GpuContext::new (spectral-gpu/src/lib.rs:21-62).  Uploads the CIE 1931 CMF
+ * and D65/A tables (cie.rs:6-8, data.rs:56-66). */
+int spx_ctx_create(int device, spx_ctx **out);
+void spx_ctx_destroy(spx_ctx *ctx);
+const char *spx_last_error(const spx_ctx *ctx);
+/* Number of compute units of the context's device. */
+int spx_ctx_num_cus(const spx_ctx *ctx);
+
+/* Camera::look_at (camera.rs:15-27), evaluated on the host.  Needs no device. */
+int spx_camera_look_at(const float origin[3], const float target[3], const float up[3],
+                       float vfov_deg, float aspect, spx_camera *out);
+
+/* ---- scene ------------------------------------------------------------------ */
+/* flatten_scene + upload (vol_photons.rs:118-183, lib.rs:91-158). */
+int spx_scene_create(spx_ctx *ctx, const spx_env *env, const spx_prim *prims, uint32_t n_prims,
+                     const spx_plane *planes, uint32_t n_planes, const spx_material *mats,
+                     uint32_t n_mats, spx_scene **out);
+void spx_scene_destroy(spx_scene *scene);
+
+/* ---- camera render: Renderer::accumulate (tracer.rs:115-141) -------------- */
+/* Adds spp hero-wavelength samples per pixel, sample indices
+ * sample_base .. sample_base+spp-1 (tracer.rs:119,127), into film_sum. */
+int spx_camera_accumulate(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam,
+                          uint32_t width, uint32_t height, uint32_t spp, uint32_t sample_base,
+                          uint32_t seed, const spx_opts *opts, uint32_t engine, float *film_sum,
+                          void *stream);
+
+/* ---- lighttrace::trace (lighttrace.rs:157-166) ------------------------------ */
+/* Photons photon_base .. photon_base+n-1; screen_sum is 3*width*height. */
+int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *screen,
+                   const spx_beam *beam, uint32_t n, uint32_t photon_base, uint32_t seed,
+                   const spx_opts *opts, uint32_t engine, float *screen_sum, void *stream);
+
+/* ---- volume (volume.rs:22-329) ---------------------------------------------- */
+/* estimator: SPX_EST_POINT (render_volumetric_scene_point), SPX_EST_BEAM
+ * (render_volumetric_scene), SPX_EST_LEGACY (render_volumetric; ignores
+ * sigma_t, g and zbuf).  zbuf: device, w*h floats, NULL = all +INF. */
+int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, const spx_beam *beam,
+               uint32_t width, uint32_t height, uint32_t n, uint32_t photon_base,
+               const spx_vol_params *vp, const float *zbuf, uint32_t seed, uint32_t estimator,
+               const spx_opts *opts, uint32_t engine, float *film_sum, void *stream);
+
+/* Per-photon parity records (GpuPhotonFilm::trace_debug, vol_photons.rs:727-779)
+ * for photons first .. first+count-1 (count <= 65536), written to host
+ * memory.  mode 0: volume point/legacy RNG order; 1: volume beam; 2:
+ * lighttrace (screen must be non-NULL).  Synchronous. */
+int spx_photon_records(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam,
+                       const spx_screen *screen, uint32_t first, uint32_t count, uint32_t seed,
+                       float max_dist, uint32_t mode, const spx_opts *opts,
+                       spx_photon_record *out_host);
+
+/* Surface rim pass + z-buffer (GpuPhotonFilm::recompute_surface,
+ * vol_photons.rs:685-714; prism_dsotm.rs:59-74).  rim_xyz (3*w*h) and
+ * depth (w*h, +INF on a miss) are device buffers, either may be NULL. */
+int spx_surface_rim(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, uint32_t width,
+                    uint32_t height, float *rim_xyz, float *depth, void *stream);
+
+/* XYZ film -> sRGB8 (xyz_to_linear_srgb + tonemap_to_u8, cie.rs:121-144)
+ * of film_sum * scale.  rgb is a device buffer of 3*w*h bytes. */
+int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint32_t height,
+                      float scale, float exposure, uint8_t *rgb, void *stream);
+
+/* ---- known-answer kernels ------------------------------------------------------ */
+/* PathRng::new(key, seed) then n next_u32 draws on the device (rng_parity.rs:4-16). */
+int spx_rng_draws(spx_ctx *ctx, uint32_t key, uint32_t seed, uint32_t n, uint32_t *out_host);
+/* Glass::n on the device (sellmeier.rs:28-44), for n wavelengths. */
+int spx_glass_n(spx_ctx *ctx, uint32_t glass, const float *lambda_host, uint32_t n,
+                float *out_host);
+
+/* ---- batched jobs (persistent launch over a device job table) ------------------ */
+#define SPX_JOB_CAMERA 0u
+#define SPX_JOB_VOLUME 1u
+#define SPX_JOB_LIGHTTRACE 2u
+
+typedef struct {
+    uint32_t kind;        /* SPX_JOB_* */
+    uint32_t width;       /* film (camera, volume) or screen (lighttrace) size */
+    uint32_t height;
+    uint32_t spp;         /* camera */
+    uint32_t sample_base; /* camera RNG counter base */
+    uint32_t n_photons;   /* volume, lighttrace */
+    uint32_t photon_base; /* photon RNG counter base */
+    uint32_t seed;
+    uint32_t estimator;   /* volume: SPX_EST_POINT only in batches */
+    uint32_t pad;
+    spx_opts opts;
+    spx_camera camera;    /* camera, volume */
+    spx_beam beam;        /* volume, lighttrace */
+    spx_screen screen;    /* lighttrace */
+    spx_vol_params vol;   /* volume */
+    const spx_scene *scene;
+    float *film;          /* device, 3*width*height, accumulated into */
+    const float *zbuf;    /* volume, device, may be NULL */
+} spx_job;
+
+typedef struct {
+    uint32_t block_size;   /* threads per block: 64..1024, multiple of 64 (0 = tuned default) */
+    uint32_t blocks;       /* persistent grid size (0 = tuned default) */
+    uint32_t chunk_paths;  /* camera work item size in paths (0 = tuned default) */
+    uint32_t chunk_photons;/* photon work item size (0 = tuned default) */
+} spx_batch_cfg;
+
+/* Runs every job of the table in ONE persistent launch on `stream`.  Work
+ * items are (job, range) chunks taken from a device queue; results equal
+ * the per-job calls at the same seeds and bases within float summation
+ * order. */
+int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg,
+                  void *stream);
+
+/* ---- device memory and streams (so callers need no HIP bindings) --------------- */
+int spx_malloc(spx_ctx *ctx, size_t bytes, void **out);
+int spx_free(spx_ctx *ctx, void *ptr);
+int spx_memcpy_h2d(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+int spx_memcpy_d2h(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+/* Fills n floats with value using a kernel (safe inside graph capture). */
+int spx_fill_f32(spx_ctx *ctx, float *dst, float value, size_t n, void *stream);
+int spx_stream_create(spx_ctx *ctx, void **out);
+int spx_stream_destroy(spx_ctx *ctx, void *stream);
+int spx_stream_sync(spx_ctx *ctx, void *stream);
+int spx_event_create(spx_ctx *ctx, void **out);
+int spx_event_destroy(spx_ctx *ctx, void *ev);
+int spx_event_record(spx_ctx *ctx, void *ev, void *stream);
+int spx_event_elapsed_ms(spx_ctx *ctx, void *ev_start, void *ev_end, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPECTRAL_HIP_H */

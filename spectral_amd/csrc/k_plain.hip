@@ -1,0 +1,236 @@
+// k_plain.hip -- the plain (one-job-per-launch) kernels and utility kernels.
+//
+// Plain camera: one thread per pixel summing its spp samples in sample order
+// (the reference's per-pixel sum, tracer.rs:120-134, and the GPU mirror's
+// thread-per-pixel launch, trace.wgsl:526-537).  Plain photons: one thread
+// per photon (volume.wgsl:542-647) with fp32 film atomics.
+#include "spx_paths.hpp"
+
+namespace spx {
+
+__global__ __launch_bounds__(256) void k_camera_plain(CamParams P) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.w * P.h) return;
+    uint32_t px = i % P.w, py = i / P.w;
+    float ax = 0.0f, ay = 0.0f, az = 0.0f;
+    for (uint32_t k = 0; k < P.spp; ++k) {
+        f3 c = camera_path(S, tbl, P, px, py, P.sample_base + k);
+        ax += c.x;
+        ay += c.y;
+        az += c.z;
+    }
+    // AccumBuffer::sum += batch (tracer.rs:135-139): one owner per pixel.
+    P.film[3 * i + 0] += ax;
+    P.film[3 * i + 1] += ay;
+    P.film[3 * i + 2] += az;
+}
+
+hipError_t launch_camera_plain(const CamParams &p, hipStream_t s) {
+    uint32_t n = p.w * p.h;
+    hipLaunchKernelGGL(k_camera_plain, dim3((n + 255) / 256), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+template <uint32_t EST>
+__global__ __launch_bounds__(256) void k_volume_plain(PhotonParams P) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    PhotonState st;
+    photon_begin(P, tbl, P.photon_base + i, st);
+    GlobalSink sink{P.film};
+    for (uint32_t b = 0; b < P.max_depth; ++b)
+        if (!volume_bounce<EST>(S, tbl, P, st, sink)) break;
+}
+
+__global__ __launch_bounds__(256) void k_lighttrace_plain(PhotonParams P) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    PhotonState st;
+    photon_begin(P, tbl, P.photon_base + i, st);
+    GlobalSink sink{P.film};
+    for (uint32_t b = 0; b < P.max_depth; ++b)
+        if (lighttrace_bounce(S, tbl, P, st, sink) != 0) break;
+}
+
+hipError_t launch_photon_plain(const PhotonParams &p, int lighttrace, hipStream_t s) {
+    dim3 grid((p.n + 255) / 256), block(256);
+    if (lighttrace) hipLaunchKernelGGL(k_lighttrace_plain, grid, block, 0, s, p);
+    else if (p.estimator == SPX_EST_POINT) hipLaunchKernelGGL(k_volume_plain<SPX_EST_POINT>, grid, block, 0, s, p);
+    else if (p.estimator == SPX_EST_BEAM) hipLaunchKernelGGL(k_volume_plain<SPX_EST_BEAM>, grid, block, 0, s, p);
+    else hipLaunchKernelGGL(k_volume_plain<SPX_EST_LEGACY>, grid, block, 0, s, p);
+    return hipGetLastError();
+}
+
+// Per-photon ray states: the same emission / intersect / scatter code as the
+// film kernels, recording (origin, dir) after every scatter
+// (cpu_simulate_photon_beam, vol_photons.rs:932-991).
+__global__ __launch_bounds__(256) void k_photon_records(PhotonParams P, uint32_t first, uint32_t count,
+                                                        uint32_t mode, spx_photon_record *out) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= count) return;
+    PhotonState st;
+    photon_begin(P, tbl, first + j, st);
+    spx_photon_record rec;
+    rec.lambda = st.lambda;
+    rec.tir_count = 0;
+    rec.pad = 0;
+    for (int a = 0; a < 9; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 4; ++c) rec.states[a][b][c] = 0.0f;
+    rec.states[0][0][0] = st.ro.x; rec.states[0][0][1] = st.ro.y; rec.states[0][0][2] = st.ro.z;
+    rec.states[0][1][0] = st.rd.x; rec.states[0][1][1] = st.rd.y; rec.states[0][1][2] = st.rd.z;
+    uint32_t ns = 1;
+    for (uint32_t b = 0; b < P.max_depth; ++b) {
+        Hit h;
+        int mi = scene_hit(S, st.ro, st.rd, h);
+        if (mode == 2u) {
+            float ss, rr, stt;
+            bool on = screen_coords(P, st.ro, st.rd, ss, rr, stt);
+            float scene_t = mi >= 0 ? h.t : INFINITY;
+            if (on && stt < scene_t) break;
+        } else if (mode == 0u) {
+            float seg_len = mi >= 0 ? h.t : P.vp.max_dist;
+            for (int k = 0; k < 4; ++k) (void)(next_f32(st.rng) * seg_len);
+        }
+        if (mi < 0) break;
+        const spx_material &m = S.mats[mi];
+        if (m.kind == SPX_MAT_DIELECTRIC) {
+            float nh = material_n(m, st.lambda, P.dispersion);
+            float n1 = h.front_face ? 1.0f : nh, n2 = h.front_face ? nh : 1.0f;
+            f3 dummy;
+            if (!refract(st.rd, h.normal, n1, n2, dummy)) rec.tir_count++;
+        }
+        photon_scatter(S, P, mi, h, st);
+        if (ns < 9) {
+            rec.states[ns][0][0] = st.ro.x; rec.states[ns][0][1] = st.ro.y; rec.states[ns][0][2] = st.ro.z;
+            rec.states[ns][1][0] = st.rd.x; rec.states[ns][1][1] = st.rd.y; rec.states[ns][1][2] = st.rd.z;
+        }
+        ++ns;
+    }
+    rec.num_states = ns;
+    out[j] = rec;
+}
+
+hipError_t launch_photon_records(const PhotonParams &p, uint32_t first, uint32_t count, uint32_t mode,
+                                 spx_photon_record *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_photon_records, dim3((count + 255) / 256), dim3(256), 0, s, p, first, count,
+                       mode, out);
+    return hipGetLastError();
+}
+
+// Surface rim + z-buffer: prism_dsotm.rs:59-74 / vol_photons.rs:1517-1538.
+__global__ __launch_bounds__(256) void k_surface_rim(const SceneData *scene, spx_camera cam, uint32_t w,
+                                                     uint32_t h, float *rim, float *depth) {
+    __shared__ SceneData S;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(scene);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&S);
+    for (uint32_t k = threadIdx.x; k < sizeof(SceneData) / 4; k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    uint32_t px = i % w, py = i / w;
+    float s = ((float)px + 0.5f) / (float)w;
+    float t = 1.0f - ((float)py + 0.5f) / (float)h;
+    f3 rd = primary_ray(cam, s, t);
+    Hit hit;
+    int mi = scene_hit(S, ld3(cam.origin), rd, hit);
+    float x = 0.f, y = 0.f, z = 0.f, dep = INFINITY;
+    if (mi >= 0) {
+        float c = fabsf(dot(hit.normal, rd));
+        c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+        float r = (1.0f - c) * (1.0f - c);
+        x = 0.015f + r * 0.45f;
+        y = 0.02f + r * 0.5f;
+        z = 0.04f + r * 0.6f;
+        dep = hit.t;
+    }
+    if (rim) {
+        rim[3 * i + 0] = x;
+        rim[3 * i + 1] = y;
+        rim[3 * i + 2] = z;
+    }
+    if (depth) depth[i] = dep;
+}
+
+hipError_t launch_surface_rim(const SceneData *scene, const spx_camera &cam, uint32_t w, uint32_t h,
+                              float *rim, float *depth, hipStream_t s) {
+    hipLaunchKernelGGL(k_surface_rim, dim3((w * h + 255) / 256), dim3(256), 0, s, scene, cam, w, h, rim,
+                       depth);
+    return hipGetLastError();
+}
+
+// xyz_to_linear_srgb + tonemap_to_u8, cie.rs:121-144.
+__global__ void k_resolve_srgb8(const float *film, uint32_t n, float scale, float exposure, uint8_t *rgb) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float x = film[3 * i] * scale, y = film[3 * i + 1] * scale, z = film[3 * i + 2] * scale;
+    float lin[3] = {3.2406f * x - 1.5372f * y - 0.4986f * z, -0.9689f * x + 1.8758f * y + 0.0415f * z,
+                    0.0557f * x - 0.2040f * y + 1.0570f * z};
+    for (int c = 0; c < 3; ++c) {
+        float v = fmaxf(lin[c] * exposure, 0.0f);
+        float mapped = v / (1.0f + v);
+        float g = mapped <= 0.0031308f ? 12.92f * mapped : 1.055f * powf(mapped, 1.0f / 2.4f) - 0.055f;
+        g = g < 0.0f ? 0.0f : (g > 1.0f ? 1.0f : g);
+        rgb[3 * i + c] = (uint8_t)(g * 255.0f + 0.5f);
+    }
+}
+
+hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float scale, float exposure,
+                                uint8_t *rgb, hipStream_t s) {
+    uint32_t n = w * h;
+    hipLaunchKernelGGL(k_resolve_srgb8, dim3((n + 255) / 256), dim3(256), 0, s, film, n, scale, exposure, rgb);
+    return hipGetLastError();
+}
+
+// Known-answer kernels ---------------------------------------------------------
+__global__ void k_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    uint32_t st = rng_new(key, seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = next_u32(st);
+}
+
+hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_rng_draws, dim3(1), dim3(64), 0, s, key, seed, n, out);
+    return hipGetLastError();
+}
+
+__global__ void k_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = glass_n(glass, lam[i]);
+}
+
+hipError_t launch_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_glass_n, dim3((n + 255) / 256), dim3(256), 0, s, glass, lam, n, out);
+    return hipGetLastError();
+}
+
+__global__ void k_fill(float *dst, float v, size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = v;
+}
+
+hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill, dim3((uint32_t)blocks), dim3(256), 0, s, dst, v, n);
+    return hipGetLastError();
+}
+
+}  // namespace spx

@@ -1,0 +1,338 @@
+// k_wave.hip -- wavefront engine: per-bounce SoA queues in HBM, wave-ballot +
+// prefix-sum compaction of live paths, scene + CIE table in LDS, photon
+// film deposits aggregated in an LDS hash before fp32 global atomics.
+//
+// Bounce schedule per chunk: [reset counters] -> first (emit + bounce 0)
+// -> bounce 1 .. max_depth-1 -> (camera) resolve.  A camera chunk holds
+// whole pixels (all spp samples of each), and the resolve kernel sums each
+// pixel's samples in sample order, so the camera film equals the plain
+// kernel's bit for bit.
+#include "spx_paths.hpp"
+#include "spx_splat.hpp"
+
+namespace spx {
+
+// Live-path compaction: one atomic per wave, slot = base + prefix(ballot).
+__device__ inline uint32_t wave_enqueue(bool alive, uint32_t *counter) {
+    uint64_t mask = __ballot(alive);
+    uint32_t lane = __lane_id();
+    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t total = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (total) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(counter, total);
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+// ---------------------------------------------------------------------------
+// Camera queues: SoA {ox,oy,oz,dx,dy,dz,rng,u_hero,tp0,slot|collapsed<<31}.
+// Live paths have zero radiance (radiance only changes at the escape that
+// ends a path) and lanes 1..3 equal lane 0 until the collapse, so ten words
+// carry the whole state.
+// ---------------------------------------------------------------------------
+struct CamQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    uint32_t *rng;
+    float *u, *tp;
+    uint32_t *slot;
+};
+
+__device__ inline void cam_store(const CamQueue &q, uint32_t i, const CamState &st, uint32_t slot) {
+    q.ox[i] = st.ro.x; q.oy[i] = st.ro.y; q.oz[i] = st.ro.z;
+    q.dx[i] = st.rd.x; q.dy[i] = st.rd.y; q.dz[i] = st.rd.z;
+    q.rng[i] = st.rng;
+    q.u[i] = st.u_hero;
+    q.tp[i] = st.tp[0];
+    q.slot[i] = slot | (st.valid == 1 ? 0x80000000u : 0u);
+}
+
+__device__ inline uint32_t cam_load(const CamQueue &q, uint32_t i, CamState &st) {
+    st.ro = mk(q.ox[i], q.oy[i], q.oz[i]);
+    st.rd = mk(q.dx[i], q.dy[i], q.dz[i]);
+    st.rng = q.rng[i];
+    st.u_hero = q.u[i];
+    st.lam0 = hero_lambda(st.u_hero, 0);
+    uint32_t s = q.slot[i];
+    bool collapsed = (s & 0x80000000u) != 0;
+    float t0 = q.tp[i];
+    st.tp[0] = t0;
+    st.tp[1] = st.tp[2] = st.tp[3] = collapsed ? 0.0f : t0;
+    st.rad[0] = st.rad[1] = st.rad[2] = st.rad[3] = 0.0f;
+    st.valid = collapsed ? 1u : (uint32_t)kHeroN;
+    return s & 0x7FFFFFFFu;
+}
+
+struct CamWave {
+    CamQueue q[2];
+    float *rx, *ry, *rz;   // per-path XYZ results, chunk-local slot order
+    uint32_t *counts;      // [33] live counts per bounce
+};
+
+__device__ inline void cam_finish(const CamWave &W, const float4 *tbl, const CamState &st, uint32_t slot) {
+    f3 c = camera_xyz(tbl, st);
+    W.rx[slot] = c.x;
+    W.ry[slot] = c.y;
+    W.rz[slot] = c.z;
+}
+
+__global__ void k_reset_counts(uint32_t *counts, uint32_t n) {
+    if (threadIdx.x < n) counts[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(256) void k_cam_first(CamParams P, CamWave W, uint32_t pix0, uint32_t npix) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    const uint32_t total = npix * P.spp;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += stride) {
+        uint32_t pixel = pix0 + j / P.spp;
+        uint32_t k = j % P.spp;
+        CamState st;
+        camera_begin(P, pixel % P.w, pixel / P.w, P.sample_base + k, st);
+        bool alive = camera_bounce(S, tbl, P, 0, st) && P.max_depth > 1;
+        uint32_t at = wave_enqueue(alive, W.counts + 1);
+        if (alive) cam_store(W.q[1], at, st, j);
+        else cam_finish(W, tbl, st, j);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_cam_bounce(CamParams P, CamWave W, uint32_t b) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    const uint32_t n_in = W.counts[b];
+    const CamQueue &qin = W.q[b & 1u];
+    const CamQueue &qout = W.q[(b + 1u) & 1u];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
+        CamState st;
+        uint32_t slot = cam_load(qin, i, st);
+        bool alive = camera_bounce(S, tbl, P, b, st) && (b + 1u < P.max_depth);
+        uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
+        if (alive) cam_store(qout, at, st, slot);
+        else cam_finish(W, tbl, st, slot);
+    }
+}
+
+// Per-pixel in-order sum of the chunk's samples (tracer.rs:125-139).
+__global__ __launch_bounds__(256) void k_cam_resolve(CamParams P, CamWave W, uint32_t pix0, uint32_t npix) {
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= npix) return;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    const uint32_t base = j * P.spp;
+    for (uint32_t k = 0; k < P.spp; ++k) {
+        ax += W.rx[base + k];
+        ay += W.ry[base + k];
+        az += W.rz[base + k];
+    }
+    uint32_t i = pix0 + j;
+    P.film[3 * i + 0] += ax;
+    P.film[3 * i + 1] += ay;
+    P.film[3 * i + 2] += az;
+}
+
+static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t camera_wavefront_scratch(uint32_t chunk) {
+    return 2 * 10 * align_up((size_t)chunk * 4) + 3 * align_up((size_t)chunk * 4) + 256;
+}
+
+static CamWave carve_cam(void *scratch, uint32_t chunk) {
+    char *p = (char *)scratch;
+    size_t a = align_up((size_t)chunk * 4);
+    CamWave W;
+    W.counts = (uint32_t *)p;
+    p += 256;
+    for (int k = 0; k < 2; ++k) {
+        CamQueue &q = W.q[k];
+        q.ox = (float *)p; p += a;
+        q.oy = (float *)p; p += a;
+        q.oz = (float *)p; p += a;
+        q.dx = (float *)p; p += a;
+        q.dy = (float *)p; p += a;
+        q.dz = (float *)p; p += a;
+        q.rng = (uint32_t *)p; p += a;
+        q.u = (float *)p; p += a;
+        q.tp = (float *)p; p += a;
+        q.slot = (uint32_t *)p; p += a;
+    }
+    W.rx = (float *)p; p += a;
+    W.ry = (float *)p; p += a;
+    W.rz = (float *)p;
+    return W;
+}
+
+static uint32_t persistent_blocks(const spx_ctx *ctx, uint32_t work, uint32_t per_cu) {
+    uint32_t cap = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * per_cu;
+    uint32_t need = (work + 255) / 256;
+    return need < cap ? (need ? need : 1) : cap;
+}
+
+hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch, uint32_t chunk,
+                                hipStream_t s) {
+    CamWave W = carve_cam(scratch, chunk);
+    uint32_t npix_total = p.w * p.h;
+    uint32_t per = chunk / p.spp;
+    if (per == 0) per = 1;  // spp > chunk: a chunk holds one pixel (needs chunk >= spp)
+    if ((uint64_t)per * p.spp > chunk) return hipErrorInvalidValue;
+    for (uint32_t pix0 = 0; pix0 < npix_total; pix0 += per) {
+        uint32_t npix = npix_total - pix0 < per ? npix_total - pix0 : per;
+        uint32_t total = npix * p.spp;
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
+        uint32_t g = persistent_blocks(ctx, total, 8);
+        hipLaunchKernelGGL(k_cam_first, dim3(g), dim3(256), 0, s, p, W, pix0, npix);
+        for (uint32_t b = 1; b < p.max_depth; ++b)
+            hipLaunchKernelGGL(k_cam_bounce, dim3(g), dim3(256), 0, s, p, W, b);
+        hipLaunchKernelGGL(k_cam_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, p, W, pix0, npix);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Photon queues: SoA {ox,oy,oz,dx,dy,dz,rng,lambda,power}.
+// ---------------------------------------------------------------------------
+struct PhQueue {
+    float *ox, *oy, *oz, *dx, *dy, *dz;
+    uint32_t *rng;
+    float *lam, *pw;
+};
+
+__device__ inline void ph_store(const PhQueue &q, uint32_t i, const PhotonState &st) {
+    q.ox[i] = st.ro.x; q.oy[i] = st.ro.y; q.oz[i] = st.ro.z;
+    q.dx[i] = st.rd.x; q.dy[i] = st.rd.y; q.dz[i] = st.rd.z;
+    q.rng[i] = st.rng;
+    q.lam[i] = st.lambda;
+    q.pw[i] = st.power;
+}
+
+__device__ inline void ph_load(const PhQueue &q, uint32_t i, PhotonState &st) {
+    st.ro = mk(q.ox[i], q.oy[i], q.oz[i]);
+    st.rd = mk(q.dx[i], q.dy[i], q.dz[i]);
+    st.rng = q.rng[i];
+    st.lambda = q.lam[i];
+    st.power = q.pw[i];
+}
+
+struct PhWave {
+    PhQueue q[2];
+    uint32_t *counts;
+};
+
+constexpr int kSplatSlots = 2048;
+
+// MODE: 0 volume point, 2 volume legacy, 3 lighttrace.
+template <int MODE>
+__device__ inline bool ph_bounce(const SceneData &S, const float4 *tbl, const PhotonParams &P,
+                                 PhotonState &st, LdsSplat<kSplatSlots> &sink) {
+    if (MODE == 3) return lighttrace_bounce(S, tbl, P, st, sink) == 0;
+    if (MODE == 2) return volume_bounce<SPX_EST_LEGACY>(S, tbl, P, st, sink);
+    return volume_bounce<SPX_EST_POINT>(S, tbl, P, st, sink);
+}
+
+#define SPX_SPLAT_LDS                                    \
+    __shared__ uint32_t sk[kSplatSlots];                 \
+    __shared__ float sv[3 * kSplatSlots];                \
+    __shared__ uint32_t sfill;                           \
+    LdsSplat<kSplatSlots> sink{sk, sv, &sfill, P.film};
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ph_first(PhotonParams P, PhWave W, uint32_t start, uint32_t count) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    SPX_SPLAT_LDS
+    stage_scene(S, P.scene, tbl, P.tbl);
+    sink.init();
+    __syncthreads();
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // Block-uniform trip count so that the flush barriers line up.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+        uint32_t j = base + threadIdx.x;
+        if (j < count) {
+            PhotonState st;
+            photon_begin(P, tbl, P.photon_base + start + j, st);
+            bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && P.max_depth > 1;
+            uint32_t at = wave_enqueue(alive, W.counts + 1);
+            if (alive) ph_store(W.q[1], at, st);
+        }
+        sink.maybe_flush();
+    }
+    sink.flush();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_ph_bounce(PhotonParams P, PhWave W, uint32_t b) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    SPX_SPLAT_LDS
+    stage_scene(S, P.scene, tbl, P.tbl);
+    sink.init();
+    __syncthreads();
+    const uint32_t n_in = W.counts[b];
+    const PhQueue &qin = W.q[b & 1u];
+    const PhQueue &qout = W.q[(b + 1u) & 1u];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
+        uint32_t i = base + threadIdx.x;
+        if (i < n_in) {
+            PhotonState st;
+            ph_load(qin, i, st);
+            bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
+            uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
+            if (alive) ph_store(qout, at, st);
+        }
+        sink.maybe_flush();
+    }
+    sink.flush();
+}
+
+size_t photon_wavefront_scratch(uint32_t chunk) { return 2 * 9 * align_up((size_t)chunk * 4) + 256; }
+
+static PhWave carve_ph(void *scratch, uint32_t chunk) {
+    char *p = (char *)scratch;
+    size_t a = align_up((size_t)chunk * 4);
+    PhWave W;
+    W.counts = (uint32_t *)p;
+    p += 256;
+    for (int k = 0; k < 2; ++k) {
+        PhQueue &q = W.q[k];
+        q.ox = (float *)p; p += a;
+        q.oy = (float *)p; p += a;
+        q.oz = (float *)p; p += a;
+        q.dx = (float *)p; p += a;
+        q.dy = (float *)p; p += a;
+        q.dz = (float *)p; p += a;
+        q.rng = (uint32_t *)p; p += a;
+        q.lam = (float *)p; p += a;
+        q.pw = (float *)p; p += a;
+    }
+    return W;
+}
+
+template <int MODE>
+static void run_ph(spx_ctx *ctx, const PhotonParams &p, PhWave W, uint32_t chunk, hipStream_t s) {
+    for (uint32_t start = 0; start < p.n; start += chunk) {
+        uint32_t count = p.n - start < chunk ? p.n - start : chunk;
+        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
+        uint32_t g = persistent_blocks(ctx, count, 4);
+        hipLaunchKernelGGL(k_ph_first<MODE>, dim3(g), dim3(256), 0, s, p, W, start, count);
+        for (uint32_t b = 1; b < p.max_depth; ++b)
+            hipLaunchKernelGGL(k_ph_bounce<MODE>, dim3(g), dim3(256), 0, s, p, W, b);
+    }
+}
+
+hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
+                                uint32_t chunk, hipStream_t s) {
+    PhWave W = carve_ph(scratch, chunk);
+    if (lighttrace) run_ph<3>(ctx, p, W, chunk, s);
+    else if (p.estimator == SPX_EST_LEGACY) run_ph<2>(ctx, p, W, chunk, s);
+    else run_ph<0>(ctx, p, W, chunk, s);
+    return hipGetLastError();
+}
+
+}  // namespace spx

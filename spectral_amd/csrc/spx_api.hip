@@ -1,0 +1,534 @@
+// spx_api.hip -- the C ABI (include/spectral_hip.h): contexts, scenes,
+// validation and dispatch to the plain / wavefront / batched kernels.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+
+#include "spx_internal.hpp"
+#include "_gen_cie.inc"
+
+using namespace spx;
+
+namespace spx {
+
+int set_error(spx_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int check_hip(spx_ctx *ctx, hipError_t e, const char *what) {
+    if (e == hipSuccess) return SPX_OK;
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_error(ctx, e == hipErrorOutOfMemory ? SPX_ENOMEM : SPX_EDEVICE, m);
+}
+
+int ensure_scratch(spx_ctx *ctx, size_t bytes, hipStream_t stream) {
+    if (ctx->scratch_bytes >= bytes) return SPX_OK;
+    if (ctx->scratch) {
+        // Work queued on any stream may still read the old buffer.
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return check_hip(ctx, e, "hipDeviceSynchronize");
+        hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_bytes = 0;
+    }
+    (void)stream;
+    size_t want = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&ctx->scratch, want);
+    if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(scratch)");
+    ctx->scratch_bytes = want;
+    return SPX_OK;
+}
+
+// Camera::project's per-camera terms, camera.rs:51-56 (camera_proj_params,
+// vol_photons.rs:100-115).
+void make_proj(const spx_camera &c, Proj &P) {
+    f3 hz = ld3(c.horizontal), vt = ld3(c.vertical), o = ld3(c.origin);
+    f3 u = normalize(hz), v = normalize(vt);
+    f3 w = o - hz * 0.5f - vt * 0.5f - ld3(c.lower_left);
+    P.o[0] = o.x; P.o[1] = o.y; P.o[2] = o.z;
+    P.u[0] = u.x; P.u[1] = u.y; P.u[2] = u.z;
+    P.v[0] = v.x; P.v[1] = v.y; P.v[2] = v.z;
+    P.w[0] = w.x; P.w[1] = w.y; P.w[2] = w.z;
+    P.vw = length(hz);
+    P.vh = length(vt);
+    P.pad[0] = P.pad[1] = 0.0f;
+}
+
+float illuminant_scale(const spx_ctx *ctx, uint32_t ill) {
+    if (ill > SPX_ILLUM_A) return 1.0f;
+    return (float)(1.0 / ctx->ill_mean[ill]);
+}
+
+const float4 *table_for(const spx_ctx *ctx, uint32_t ill) {
+    return ctx->tbl_dev + (ill == SPX_ILLUM_A ? kTblN : 0);
+}
+
+}  // namespace spx
+
+// ---------------------------------------------------------------------------
+// Table::parse (cie.rs:24-45) over the embedded CSV text.
+static int parse_table(const char *text, int cols, float *dst /* [71][cols] */) {
+    const char *p = text;
+    int n = 0;
+    while (*p) {
+        while (*p == ' ' || *p == '\t' || *p == '\r' || *p == '\n') ++p;
+        if (!*p) break;
+        if (n >= kTblN) return -1;
+        char *e;
+        (void)strtof(p, &e);
+        p = e;
+        for (int c = 0; c < cols; ++c) {
+            while (*p == ',' || *p == ' ') ++p;
+            dst[n * cols + c] = strtof(p, &e);
+            p = e;
+        }
+        while (*p && *p != '\n') ++p;
+        ++n;
+    }
+    return n == kTblN ? 0 : -2;
+}
+
+static bool valid_opts(const spx_opts *o, bool camera) {
+    if (o->max_depth < 1 || o->max_depth > 32) return false;
+    if (o->illuminant > SPX_ILLUM_E) return false;
+    if (o->dispersion > 1) return false;
+    (void)camera;
+    return true;
+}
+
+static spx_opts default_opts(bool camera) {
+    spx_opts o;
+    o.illuminant = camera ? SPX_ILLUM_D65 : SPX_ILLUM_E;
+    o.dispersion = 1;
+    o.max_depth = 8;
+    o.rr_depth = 0;
+    return o;
+}
+
+static bool finite3(const float *v) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
+
+extern "C" {
+
+int spx_ctx_create(int device, spx_ctx **out) {
+    if (!out) return SPX_EINVAL;
+    *out = nullptr;
+    spx_ctx *ctx = new spx_ctx();
+    ctx->device = device;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || device < 0 || device >= ndev) {
+        delete ctx;
+        return e != hipSuccess ? SPX_EDEVICE : SPX_EINVAL;
+    }
+    if ((e = hipSetDevice(device)) != hipSuccess) {
+        delete ctx;
+        return SPX_EDEVICE;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    float cmf[kTblN][3], d65[kTblN], a[kTblN];
+    if (parse_table(kCmfCsv, 3, &cmf[0][0]) || parse_table(kD65Csv, 1, d65) || parse_table(kACsv, 1, a)) {
+        delete ctx;
+        return SPX_EINVAL;
+    }
+    double s65 = 0.0, sa = 0.0;
+    for (int i = 0; i < kTblN; ++i) {
+        for (int t = 0; t < 2; ++t) {
+            ctx->tbl_host[t][i][0] = cmf[i][0];
+            ctx->tbl_host[t][i][1] = cmf[i][1];
+            ctx->tbl_host[t][i][2] = cmf[i][2];
+        }
+        ctx->tbl_host[0][i][3] = d65[i];
+        ctx->tbl_host[1][i][3] = a[i];
+        s65 += (double)d65[i];
+        sa += (double)a[i];
+    }
+    ctx->ill_mean[0] = s65 / (double)kTblN;
+    ctx->ill_mean[1] = sa / (double)kTblN;
+    if ((e = hipMalloc(&ctx->tbl_dev, sizeof(ctx->tbl_host))) != hipSuccess ||
+        (e = hipMemcpy(ctx->tbl_dev, ctx->tbl_host, sizeof(ctx->tbl_host), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        if (ctx->tbl_dev) hipFree(ctx->tbl_dev);
+        delete ctx;
+        return e == hipErrorOutOfMemory ? SPX_ENOMEM : SPX_EDEVICE;
+    }
+    *out = ctx;
+    return SPX_OK;
+}
+
+void spx_ctx_destroy(spx_ctx *ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    hipDeviceSynchronize();
+    if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->tbl_dev) hipFree(ctx->tbl_dev);
+    delete ctx;
+}
+
+const char *spx_last_error(const spx_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int spx_camera_look_at(const float origin[3], const float target[3], const float up[3], float vfov_deg,
+                       float aspect, spx_camera *out) {
+    if (!origin || !target || !up || !out) return SPX_EINVAL;
+    const float rads_per_deg = 3.14159265358979323846f / 180.0f;  // f32::to_radians
+    f3 o = ld3(origin), tg = ld3(target), up3 = ld3(up);
+    float theta = vfov_deg * rads_per_deg;
+    float hh = tanf(theta / 2.0f);
+    float viewport_h = 2.0f * hh;
+    float viewport_w = aspect * viewport_h;
+    f3 w = normalize(o - tg);
+    f3 u = normalize(cross(up3, w));
+    f3 v = cross(w, u);
+    f3 horizontal = u * viewport_w;
+    f3 vertical = v * viewport_h;
+    f3 ll = o - horizontal / 2.0f - vertical / 2.0f - w;
+    const f3 *src[4] = {&o, &ll, &horizontal, &vertical};
+    float *dst[4] = {out->origin, out->lower_left, out->horizontal, out->vertical};
+    for (int k = 0; k < 4; ++k) {
+        dst[k][0] = src[k]->x;
+        dst[k][1] = src[k]->y;
+        dst[k][2] = src[k]->z;
+    }
+    if (!finite3(out->origin) || !finite3(out->lower_left) || !finite3(out->horizontal) ||
+        !finite3(out->vertical))
+        return SPX_EINVAL;
+    return SPX_OK;
+}
+
+int spx_ctx_num_cus(const spx_ctx *ctx) { return ctx ? ctx->num_cus : 0; }
+
+int spx_scene_create(spx_ctx *ctx, const spx_env *env, const spx_prim *prims, uint32_t n_prims,
+                     const spx_plane *planes, uint32_t n_planes, const spx_material *mats, uint32_t n_mats,
+                     spx_scene **out) {
+    if (!ctx || !out || !env) return set_error(ctx, SPX_EINVAL, "spx_scene_create: null argument");
+    *out = nullptr;
+    if (n_prims > SPX_MAX_PRIMS || n_planes > SPX_MAX_PLANES || n_mats > SPX_MAX_MATERIALS)
+        return set_error(ctx, SPX_ECAP, "spx_scene_create: scene exceeds the LDS capacity");
+    if ((n_prims && !prims) || (n_planes && !planes) || (n_mats && !mats))
+        return set_error(ctx, SPX_EINVAL, "spx_scene_create: null array with nonzero count");
+    for (uint32_t i = 0; i < n_prims; ++i) {
+        const spx_prim &p = prims[i];
+        if (p.kind > SPX_PRIM_SOLID || p.material >= n_mats)
+            return set_error(ctx, SPX_EINVAL, "spx_scene_create: bad primitive kind or material");
+        if (p.kind == SPX_PRIM_SOLID && ((uint64_t)p.plane_start + p.plane_count > n_planes))
+            return set_error(ctx, SPX_EINVAL, "spx_scene_create: plane range out of bounds");
+    }
+    for (uint32_t i = 0; i < n_mats; ++i)
+        if (mats[i].kind > SPX_MAT_DIELECTRIC || mats[i].glass > SPX_GLASS_WATER)
+            return set_error(ctx, SPX_EINVAL, "spx_scene_create: bad material");
+    spx_scene *sc = new spx_scene();
+    sc->ctx = ctx;
+    memset(&sc->host, 0, sizeof sc->host);
+    sc->host.n_prims = n_prims;
+    sc->host.n_planes = n_planes;
+    sc->host.n_mats = n_mats;
+    sc->host.env = *env;
+    if (n_prims) memcpy(sc->host.prims, prims, n_prims * sizeof(spx_prim));
+    if (n_planes) memcpy(sc->host.planes, planes, n_planes * sizeof(spx_plane));
+    if (n_mats) memcpy(sc->host.mats, mats, n_mats * sizeof(spx_material));
+    hipError_t e = hipMalloc(&sc->dev, sizeof(SceneData));
+    if (e == hipSuccess) e = hipMemcpy(sc->dev, &sc->host, sizeof(SceneData), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (sc->dev) hipFree(sc->dev);
+        delete sc;
+        return check_hip(ctx, e, "spx_scene_create upload");
+    }
+    *out = sc;
+    return SPX_OK;
+}
+
+void spx_scene_destroy(spx_scene *scene) {
+    if (!scene) return;
+    hipDeviceSynchronize();
+    if (scene->dev) hipFree(scene->dev);
+    delete scene;
+}
+
+static int check_common(spx_ctx *ctx, const spx_scene *scene, const char *who) {
+    if (!ctx) return SPX_EINVAL;
+    if (!scene || scene->ctx != ctx)
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": scene is null or from another context");
+    return SPX_OK;
+}
+
+static int check_film(spx_ctx *ctx, const float *film, uint32_t w, uint32_t h, const char *who) {
+    if (!film) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null film");
+    if (w == 0 || h == 0) return set_error(ctx, SPX_EINVAL, std::string(who) + ": zero-size film");
+    if ((uint64_t)w * h > (1ull << 28))
+        return set_error(ctx, SPX_ECAP, std::string(who) + ": film larger than 2^28 pixels");
+    return SPX_OK;
+}
+
+int spx_camera_accumulate(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, uint32_t width,
+                          uint32_t height, uint32_t spp, uint32_t sample_base, uint32_t seed,
+                          const spx_opts *opts, uint32_t engine, float *film_sum, void *stream) {
+    int rc = check_common(ctx, scene, "spx_camera_accumulate");
+    if (rc) return rc;
+    if (!cam) return set_error(ctx, SPX_EINVAL, "spx_camera_accumulate: null camera");
+    if ((rc = check_film(ctx, film_sum, width, height, "spx_camera_accumulate"))) return rc;
+    spx_opts o = opts ? *opts : default_opts(true);
+    if (!valid_opts(&o, true)) return set_error(ctx, SPX_EINVAL, "spx_camera_accumulate: bad options");
+    if (!finite3(cam->origin) || !finite3(cam->lower_left) || !finite3(cam->horizontal) ||
+        !finite3(cam->vertical))
+        return set_error(ctx, SPX_EINVAL, "spx_camera_accumulate: non-finite camera");
+    if (spp == 0) return SPX_OK;
+    CamParams p;
+    memset(&p, 0, sizeof p);
+    p.cam = *cam;
+    p.w = width; p.h = height; p.spp = spp; p.sample_base = sample_base; p.seed = seed;
+    p.illuminant = o.illuminant; p.dispersion = o.dispersion; p.max_depth = o.max_depth;
+    p.rr_depth = o.rr_depth;
+    p.scene = scene->dev;
+    p.tbl = table_for(ctx, o.illuminant);
+    p.film = film_sum;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (engine == SPX_ENGINE_PLAIN) {
+        e = launch_camera_plain(p, s);
+    } else if (engine == SPX_ENGINE_WAVEFRONT) {
+        uint32_t chunk = spp > (1u << 21) ? spp : (1u << 21);
+        if ((rc = ensure_scratch(ctx, camera_wavefront_scratch(chunk), s))) return rc;
+        e = run_camera_wavefront(ctx, p, ctx->scratch, chunk, s);
+    } else {
+        return set_error(ctx, SPX_EINVAL, "spx_camera_accumulate: unknown engine");
+    }
+    return check_hip(ctx, e, "spx_camera_accumulate launch");
+}
+
+static int photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_opts *opts,
+                         uint32_t n, uint32_t photon_base, uint32_t seed, PhotonParams &p, const char *who) {
+    if (!beam) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null beam");
+    if (!finite3(beam->corner) || !finite3(beam->u) || !finite3(beam->v) || !finite3(beam->dir))
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite beam");
+    spx_opts o = opts ? *opts : default_opts(false);
+    if (!valid_opts(&o, false)) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad options");
+    memset(&p, 0, sizeof p);
+    p.beam = *beam;
+    p.n = n;
+    p.photon_base = photon_base;
+    p.seed = seed;
+    p.illuminant = o.illuminant;
+    p.dispersion = o.dispersion;
+    p.max_depth = o.max_depth;
+    p.ill_scale = illuminant_scale(ctx, o.illuminant);
+    p.scene = scene->dev;
+    p.tbl = table_for(ctx, o.illuminant);
+    return SPX_OK;
+}
+
+static void set_screen(PhotonParams &p, const spx_screen &scr) {
+    // Screen::new, lighttrace.rs:27-34
+    f3 u = ld3(scr.u), v = ld3(scr.v), n = normalize(cross(u, v));
+    memcpy(p.scr_corner, scr.corner, sizeof p.scr_corner);
+    memcpy(p.scr_u, scr.u, sizeof p.scr_u);
+    memcpy(p.scr_v, scr.v, sizeof p.scr_v);
+    p.scr_n[0] = n.x; p.scr_n[1] = n.y; p.scr_n[2] = n.z;
+    p.scr_uu = dot(u, u);
+    p.scr_vv = dot(v, v);
+    p.scr_w = scr.width;
+    p.scr_h = scr.height;
+}
+
+int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *screen, const spx_beam *beam,
+                   uint32_t n, uint32_t photon_base, uint32_t seed, const spx_opts *opts, uint32_t engine,
+                   float *screen_sum, void *stream) {
+    int rc = check_common(ctx, scene, "spx_lighttrace");
+    if (rc) return rc;
+    if (!screen) return set_error(ctx, SPX_EINVAL, "spx_lighttrace: null screen");
+    if ((rc = check_film(ctx, screen_sum, screen->width, screen->height, "spx_lighttrace"))) return rc;
+    PhotonParams p;
+    if ((rc = photon_params(ctx, scene, beam, opts, n, photon_base, seed, p, "spx_lighttrace"))) return rc;
+    set_screen(p, *screen);
+    p.film = screen_sum;
+    if (n == 0) return SPX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (engine == SPX_ENGINE_PLAIN) {
+        e = launch_photon_plain(p, 1, s);
+    } else if (engine == SPX_ENGINE_WAVEFRONT) {
+        uint32_t chunk = 1u << 22;
+        if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
+        e = run_photon_wavefront(ctx, p, 1, ctx->scratch, chunk, s);
+    } else {
+        return set_error(ctx, SPX_EINVAL, "spx_lighttrace: unknown engine");
+    }
+    return check_hip(ctx, e, "spx_lighttrace launch");
+}
+
+int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, const spx_beam *beam,
+               uint32_t width, uint32_t height, uint32_t n, uint32_t photon_base, const spx_vol_params *vp,
+               const float *zbuf, uint32_t seed, uint32_t estimator, const spx_opts *opts, uint32_t engine,
+               float *film_sum, void *stream) {
+    int rc = check_common(ctx, scene, "spx_volume");
+    if (rc) return rc;
+    if (!cam || !vp) return set_error(ctx, SPX_EINVAL, "spx_volume: null camera or parameters");
+    if ((rc = check_film(ctx, film_sum, width, height, "spx_volume"))) return rc;
+    if (estimator > SPX_EST_LEGACY) return set_error(ctx, SPX_EINVAL, "spx_volume: unknown estimator");
+    if (!isfinite(vp->sigma_s) || !isfinite(vp->sigma_t) || !isfinite(vp->g) || !(vp->max_dist > 0.0f) ||
+        !isfinite(vp->max_dist))
+        return set_error(ctx, SPX_EINVAL, "spx_volume: bad volume parameters");
+    PhotonParams p;
+    if ((rc = photon_params(ctx, scene, beam, opts, n, photon_base, seed, p, "spx_volume"))) return rc;
+    make_proj(*cam, p.proj);
+    p.vp = *vp;
+    p.w = width;
+    p.h = height;
+    p.estimator = estimator;
+    p.zbuf = zbuf;
+    p.film = film_sum;
+    if (n == 0) return SPX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e;
+    if (engine == SPX_ENGINE_PLAIN || estimator == SPX_EST_BEAM) {
+        e = launch_photon_plain(p, 0, s);
+    } else if (engine == SPX_ENGINE_WAVEFRONT) {
+        uint32_t chunk = 1u << 22;
+        if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
+        e = run_photon_wavefront(ctx, p, 0, ctx->scratch, chunk, s);
+    } else {
+        return set_error(ctx, SPX_EINVAL, "spx_volume: unknown engine");
+    }
+    return check_hip(ctx, e, "spx_volume launch");
+}
+
+int spx_photon_records(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_screen *screen,
+                       uint32_t first, uint32_t count, uint32_t seed, float max_dist, uint32_t mode,
+                       const spx_opts *opts, spx_photon_record *out_host) {
+    int rc = check_common(ctx, scene, "spx_photon_records");
+    if (rc) return rc;
+    if (!out_host) return set_error(ctx, SPX_EINVAL, "spx_photon_records: null output");
+    if (count > 65536) return set_error(ctx, SPX_ECAP, "spx_photon_records: count > 65536");
+    if (mode > 2 || (mode == 2 && !screen)) return set_error(ctx, SPX_EINVAL, "spx_photon_records: bad mode");
+    PhotonParams p;
+    if ((rc = photon_params(ctx, scene, beam, opts, count, first, seed, p, "spx_photon_records"))) return rc;
+    if (p.max_depth > 8) return set_error(ctx, SPX_ECAP, "spx_photon_records: max_depth > 8");
+    if (screen) set_screen(p, *screen);
+    p.vp.max_dist = max_dist;
+    if (count == 0) return SPX_OK;
+    spx_photon_record *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, count * sizeof(spx_photon_record));
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_photon_records alloc");
+    e = launch_photon_records(p, first, count, mode, dev, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out_host, dev, count * sizeof(spx_photon_record), hipMemcpyDeviceToHost);
+    hipFree(dev);
+    return check_hip(ctx, e, "spx_photon_records");
+}
+
+int spx_surface_rim(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, uint32_t width,
+                    uint32_t height, float *rim_xyz, float *depth, void *stream) {
+    int rc = check_common(ctx, scene, "spx_surface_rim");
+    if (rc) return rc;
+    if (!cam || width == 0 || height == 0) return set_error(ctx, SPX_EINVAL, "spx_surface_rim: bad arguments");
+    if (!rim_xyz && !depth) return SPX_OK;
+    return check_hip(ctx, launch_surface_rim(scene->dev, *cam, width, height, rim_xyz, depth, (hipStream_t)stream),
+                     "spx_surface_rim");
+}
+
+int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint32_t height, float scale,
+                      float exposure, uint8_t *rgb, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    if (!film_sum || !rgb || width == 0 || height == 0)
+        return set_error(ctx, SPX_EINVAL, "spx_resolve_srgb8: bad arguments");
+    return check_hip(ctx, launch_resolve_srgb8(film_sum, width, height, scale, exposure, rgb, (hipStream_t)stream),
+                     "spx_resolve_srgb8");
+}
+
+int spx_rng_draws(spx_ctx *ctx, uint32_t key, uint32_t seed, uint32_t n, uint32_t *out_host) {
+    if (!ctx) return SPX_EINVAL;
+    if (!out_host || n == 0 || n > 1u << 20) return set_error(ctx, SPX_EINVAL, "spx_rng_draws: bad arguments");
+    uint32_t *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, n * sizeof(uint32_t));
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_rng_draws alloc");
+    e = launch_rng_draws(key, seed, n, dev, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out_host, dev, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    hipFree(dev);
+    return check_hip(ctx, e, "spx_rng_draws");
+}
+
+int spx_glass_n(spx_ctx *ctx, uint32_t glass, const float *lambda_host, uint32_t n, float *out_host) {
+    if (!ctx) return SPX_EINVAL;
+    if (!lambda_host || !out_host || n == 0 || glass > SPX_GLASS_WATER)
+        return set_error(ctx, SPX_EINVAL, "spx_glass_n: bad arguments");
+    float *dev = nullptr;
+    hipError_t e = hipMalloc(&dev, 2 * (size_t)n * sizeof(float));
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_glass_n alloc");
+    e = hipMemcpy(dev, lambda_host, n * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_glass_n(glass, dev, n, dev + n, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out_host, dev + n, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(dev);
+    return check_hip(ctx, e, "spx_glass_n");
+}
+
+// ---- memory / streams / events ----------------------------------------------
+int spx_malloc(spx_ctx *ctx, size_t bytes, void **out) {
+    if (!ctx || !out || bytes == 0) return set_error(ctx, SPX_EINVAL, "spx_malloc: bad arguments");
+    *out = nullptr;
+    return check_hip(ctx, hipMalloc(out, bytes), "spx_malloc");
+}
+int spx_free(spx_ctx *ctx, void *ptr) {
+    if (!ctx) return SPX_EINVAL;
+    if (!ptr) return SPX_OK;
+    return check_hip(ctx, hipFree(ptr), "spx_free");
+}
+int spx_memcpy_h2d(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return set_error(ctx, SPX_EINVAL, "spx_memcpy_h2d: null");
+    if (!bytes) return SPX_OK;
+    hipError_t e = stream ? hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream)
+                          : hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+    return check_hip(ctx, e, "spx_memcpy_h2d");
+}
+int spx_memcpy_d2h(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream) {
+    if (!ctx || (!dst && bytes) || (!src && bytes)) return set_error(ctx, SPX_EINVAL, "spx_memcpy_d2h: null");
+    if (!bytes) return SPX_OK;
+    hipError_t e = stream ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream)
+                          : hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+    return check_hip(ctx, e, "spx_memcpy_d2h");
+}
+int spx_fill_f32(spx_ctx *ctx, float *dst, float value, size_t n, void *stream) {
+    if (!ctx || (!dst && n)) return set_error(ctx, SPX_EINVAL, "spx_fill_f32: null");
+    return check_hip(ctx, launch_fill(dst, value, n, (hipStream_t)stream), "spx_fill_f32");
+}
+int spx_stream_create(spx_ctx *ctx, void **out) {
+    if (!ctx || !out) return SPX_EINVAL;
+    hipStream_t s = nullptr;
+    int rc = check_hip(ctx, hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "spx_stream_create");
+    *out = s;
+    return rc;
+}
+int spx_stream_destroy(spx_ctx *ctx, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    if (!stream) return SPX_OK;
+    return check_hip(ctx, hipStreamDestroy((hipStream_t)stream), "spx_stream_destroy");
+}
+int spx_stream_sync(spx_ctx *ctx, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    return check_hip(ctx, hipStreamSynchronize((hipStream_t)stream), "spx_stream_sync");
+}
+int spx_event_create(spx_ctx *ctx, void **out) {
+    if (!ctx || !out) return SPX_EINVAL;
+    hipEvent_t ev = nullptr;
+    int rc = check_hip(ctx, hipEventCreate(&ev), "spx_event_create");
+    *out = ev;
+    return rc;
+}
+int spx_event_destroy(spx_ctx *ctx, void *ev) {
+    if (!ctx) return SPX_EINVAL;
+    if (!ev) return SPX_OK;
+    return check_hip(ctx, hipEventDestroy((hipEvent_t)ev), "spx_event_destroy");
+}
+int spx_event_record(spx_ctx *ctx, void *ev, void *stream) {
+    if (!ctx || !ev) return SPX_EINVAL;
+    return check_hip(ctx, hipEventRecord((hipEvent_t)ev, (hipStream_t)stream), "spx_event_record");
+}
+int spx_event_elapsed_ms(spx_ctx *ctx, void *a, void *b, float *ms) {
+    if (!ctx || !a || !b || !ms) return SPX_EINVAL;
+    hipError_t e = hipEventSynchronize((hipEvent_t)b);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b);
+    return check_hip(ctx, e, "spx_event_elapsed_ms");
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// spx_internal.hpp -- host-side internals shared by the kernel and ABI units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "spectral_hip.h"
+#include "spx_device.hpp"
+
+namespace spx {
+
+// Per-launch parameter blocks (passed by value as kernel arguments).
+struct CamParams {
+    spx_camera cam;
+    uint32_t w, h, spp, sample_base;
+    uint32_t seed, illuminant, dispersion, max_depth;
+    uint32_t rr_depth, pad0, pad1, pad2;
+    const SceneData *scene;
+    const float4 *tbl;  // 71 rows {xbar,ybar,zbar,S} for the job's illuminant
+    float *film;
+};
+
+struct PhotonParams {
+    spx_beam beam;
+    Proj proj;             // volume: Camera::project terms
+    float scr_corner[3], scr_uu;
+    float scr_u[3], scr_vv;
+    float scr_v[3], pad0;
+    float scr_n[3], pad1;  // lighttrace: Screen::normal
+    uint32_t scr_w, scr_h, pad2, pad3;
+    spx_vol_params vp;
+    uint32_t w, h, n, photon_base;
+    uint32_t seed, estimator, illuminant, dispersion;
+    uint32_t max_depth, pad4;
+    float ill_scale;       // photon power weight S(lambda)/mean S (1 for equal energy)
+    float pad5;
+    const SceneData *scene;
+    const float4 *tbl;
+    const float *zbuf;     // may be null (= +INF)
+    float *film;
+};
+
+}  // namespace spx
+
+struct spx_ctx {
+    int device = 0;
+    int num_cus = 0;
+    float4 *tbl_dev = nullptr;  // [2][71]: D65 rows then A rows (w = S)
+    float tbl_host[2][spx::kTblN][4];
+    double ill_mean[2];
+    std::string last_error;
+    void *scratch = nullptr;   // grow-only work buffer for queues / job tables
+    size_t scratch_bytes = 0;
+    hipStream_t scratch_stream = nullptr;
+};
+
+struct spx_scene {
+    spx_ctx *ctx = nullptr;
+    spx::SceneData host;
+    spx::SceneData *dev = nullptr;
+};
+
+namespace spx {
+
+int set_error(spx_ctx *ctx, int code, const std::string &msg);
+int check_hip(spx_ctx *ctx, hipError_t e, const char *what);
+// Ensures ctx->scratch holds at least `bytes`; synchronises `stream` first
+// when it has to grow.
+int ensure_scratch(spx_ctx *ctx, size_t bytes, hipStream_t stream);
+
+void make_proj(const spx_camera &c, Proj &P);
+float illuminant_scale(const spx_ctx *ctx, uint32_t ill);
+const float4 *table_for(const spx_ctx *ctx, uint32_t ill);
+
+// Kernel launchers (return hipGetLastError()).
+hipError_t launch_camera_plain(const CamParams &p, hipStream_t s);
+hipError_t launch_photon_plain(const PhotonParams &p, int lighttrace, hipStream_t s);
+hipError_t launch_photon_records(const PhotonParams &p, uint32_t first, uint32_t count,
+                                 uint32_t mode, spx_photon_record *out, hipStream_t s);
+hipError_t launch_surface_rim(const SceneData *scene, const spx_camera &cam, uint32_t w, uint32_t h,
+                              float *rim, float *depth, hipStream_t s);
+hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float scale,
+                                float exposure, uint8_t *rgb, hipStream_t s);
+hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out, hipStream_t s);
+hipError_t launch_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out, hipStream_t s);
+hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s);
+
+// Wavefront engine (per-bounce SoA queues).
+size_t camera_wavefront_scratch(uint32_t chunk);
+hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch, uint32_t chunk,
+                                hipStream_t s);
+size_t photon_wavefront_scratch(uint32_t chunk);
+hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
+                                uint32_t chunk, hipStream_t s);
+
+}  // namespace spx

@@ -1,0 +1,357 @@
+// spx_paths.hpp -- per-path / per-photon transport routines shared by the
+// plain, wavefront and batched kernels.  Each restates one reference loop.
+#pragma once
+
+#include "spx_internal.hpp"
+
+namespace spx {
+
+// Cooperative copy of the scene and the 71-row CIE table into LDS.
+__device__ inline void stage_scene(SceneData &dst, const SceneData *src, float4 *tbl,
+                                   const float4 *tsrc) {
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(&dst);
+    constexpr uint32_t words = sizeof(SceneData) / 4;
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) d[k] = s[k];
+    for (uint32_t k = threadIdx.x; k < (uint32_t)kTblN; k += blockDim.x) tbl[k] = tsrc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Camera path: CpuTracer::sample_pixel, tracer.rs:44-111 (+ optional
+// Russian roulette before bounce b >= rr_depth: survive with
+// q = clamp(max throughput, 0.05, 0.95), throughput /= q).
+// ---------------------------------------------------------------------------
+struct CamState {
+    f3 ro, rd;
+    uint32_t rng;
+    float lam0;     // hero wavelength (lane 0)
+    float u_hero;   // the comb draw; lane k is hero_lambda(u_hero, k)
+    float tp[4];
+    float rad[4];
+    uint32_t valid;
+};
+
+__device__ inline void camera_begin(const CamParams &P, uint32_t px, uint32_t py, uint32_t sidx,
+                                    CamState &st) {
+    uint32_t pixel = py * P.w + px;
+    st.rng = rng_new(mix2(pixel, sidx), P.seed);
+    float s = ((float)px + next_f32(st.rng)) / (float)P.w;
+    float t = 1.0f - ((float)py + next_f32(st.rng)) / (float)P.h;
+    st.u_hero = next_f32(st.rng);
+    st.lam0 = hero_lambda(st.u_hero, 0);
+#pragma unroll
+    for (int k = 0; k < kHeroN; ++k) {
+        st.tp[k] = 1.0f;
+        st.rad[k] = 0.0f;
+    }
+    st.valid = kHeroN;
+    st.ro = ld3(P.cam.origin);
+    st.rd = primary_ray(P.cam, s, t);
+}
+
+// One bounce-loop iteration b.  Returns false when the path has ended.
+__device__ inline bool camera_bounce(const SceneData &S, const float4 *tbl, const CamParams &P,
+                                     uint32_t b, CamState &st) {
+    if (P.rr_depth && b >= P.rr_depth) {
+        float q = fmaxf(fmaxf(fmaxf(st.tp[0], st.tp[1]), st.tp[2]), st.tp[3]);
+        q = q < 0.05f ? 0.05f : (q > 0.95f ? 0.95f : q);
+        if (next_f32(st.rng) >= q) return false;
+#pragma unroll
+        for (int k = 0; k < kHeroN; ++k) st.tp[k] = st.tp[k] / q;
+    }
+    Hit h;
+    int mi = scene_hit(S, st.ro, st.rd, h);
+    if (mi < 0) {
+        float bg = background_radiance(S.env, st.rd);
+        bool eq = P.illuminant == SPX_ILLUM_E;
+#pragma unroll
+        for (int k = 0; k < kHeroN; ++k) {
+            float sp = ill_lookup(tbl, hero_lambda(st.u_hero, k), eq);
+            st.rad[k] += st.tp[k] * bg * sp;
+        }
+        return false;
+    }
+    const spx_material &m = S.mats[mi];
+    float nh = material_n(m, st.lam0, P.dispersion);
+    f3 nd;
+    bool tir;
+    float w = scatter(m, st.rd, h, nh, st.rng, nd, tir);
+#pragma unroll
+    for (int k = 0; k < kHeroN; ++k) st.tp[k] *= w;
+    if (m.kind == SPX_MAT_DIELECTRIC && st.valid > 1) {
+        st.tp[1] = st.tp[2] = st.tp[3] = 0.0f;
+        st.valid = 1;
+    }
+    st.ro = h.point;
+    st.rd = nd;
+    return true;
+}
+
+// XYZ of a finished path, tracer.rs:102-110.
+__device__ inline f3 camera_xyz(const float4 *tbl, const CamState &st) {
+    const float pdf = 1.0f / kLambdaRange;
+    f3 xyz = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kHeroN; ++k) {
+        f3 c = cmf_lookup(tbl, hero_lambda(st.u_hero, k));
+        float wk = st.rad[k] / pdf / (float)st.valid;
+        xyz.x += c.x * wk;
+        xyz.y += c.y * wk;
+        xyz.z += c.z * wk;
+    }
+    return xyz;
+}
+
+__device__ inline f3 camera_path(const SceneData &S, const float4 *tbl, const CamParams &P,
+                                 uint32_t px, uint32_t py, uint32_t sidx) {
+    CamState st;
+    camera_begin(P, px, py, sidx, st);
+    for (uint32_t b = 0; b < P.max_depth; ++b)
+        if (!camera_bounce(S, tbl, P, b, st)) break;
+    return camera_xyz(tbl, st);
+}
+
+// ---------------------------------------------------------------------------
+// Photon emission: lighttrace.rs:159-161 / volume.rs:119-124.
+// ---------------------------------------------------------------------------
+struct PhotonState {
+    f3 ro, rd;
+    uint32_t rng;
+    float lambda;
+    float power;
+};
+
+__device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, uint32_t i,
+                                    PhotonState &st) {
+    st.rng = rng_new(i, P.seed);
+    float a = next_f32(st.rng);
+    float b = next_f32(st.rng);
+    st.ro = ld3(P.beam.corner) + ld3(P.beam.u) * a + ld3(P.beam.v) * b;
+    st.lambda = kLambdaMin + kLambdaRange * next_f32(st.rng);
+    st.rd = ld3(P.beam.dir);
+    st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
+}
+
+// Scatter at a hit (the tail of every photon loop).  Returns tir.
+__device__ inline bool photon_scatter(const SceneData &S, const PhotonParams &P, int mi, const Hit &h,
+                                      PhotonState &st) {
+    const spx_material &m = S.mats[mi];
+    float nh = material_n(m, st.lambda, P.dispersion);
+    f3 nd;
+    bool tir;
+    float w = scatter(m, st.rd, h, nh, st.rng, nd, tir);
+    st.power *= w;
+    st.ro = h.point;
+    st.rd = nd;
+    return tir;
+}
+
+// Screen::screen_coords, lighttrace.rs:42-60 (normal, |u|^2, |v|^2 hoisted).
+__device__ inline bool screen_coords(const PhotonParams &P, f3 o, f3 d, float &s, float &r, float &t) {
+    f3 n = ld3(P.scr_n);
+    float denom = dot(d, n);
+    if (fabsf(denom) < 1e-8f) return false;
+    float tt = dot(ld3(P.scr_corner) - o, n) / denom;
+    if (tt <= 1e-4f) return false;
+    f3 p = o + d * tt;
+    f3 rel = p - ld3(P.scr_corner);
+    float ss = dot(rel, ld3(P.scr_u)) / P.scr_uu;
+    float rr = dot(rel, ld3(P.scr_v)) / P.scr_vv;
+    if (ss >= 0.0f && ss < 1.0f && rr >= 0.0f && rr < 1.0f) {
+        s = ss;
+        r = rr;
+        t = tt;
+        return true;
+    }
+    return false;
+}
+
+// Screen::splat (texel_bilinear + 4-texel deposit), lighttrace.rs:65-91.
+template <class Sink>
+__device__ inline void screen_splat(const PhotonParams &P, const float4 *tbl, float s, float r,
+                                    float lambda, float power, Sink &sink) {
+    f3 c = cmf_lookup(tbl, lambda);
+    float W = (float)P.scr_w, H = (float)P.scr_h;
+    float fx = s * W - 0.5f;
+    fx = fx < 0.0f ? 0.0f : (fx > W - 1.0f ? W - 1.0f : fx);
+    float fy = (1.0f - r) * H - 0.5f;
+    fy = fy < 0.0f ? 0.0f : (fy > H - 1.0f ? H - 1.0f : fy);
+    uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
+    uint32_t x1 = x0 + 1 < P.scr_w - 1 ? x0 + 1 : P.scr_w - 1;
+    uint32_t y1 = y0 + 1 < P.scr_h - 1 ? y0 + 1 : P.scr_h - 1;
+    float dx = fx - (float)x0, dy = fy - (float)y0;
+    float w00 = (1.0f - dx) * (1.0f - dy), w10 = dx * (1.0f - dy);
+    float w01 = (1.0f - dx) * dy, w11 = dx * dy;
+    f3 cp = c * power;
+    sink.add(y0 * P.scr_w + x0, cp * w00);
+    sink.add(y0 * P.scr_w + x1, cp * w10);
+    sink.add(y1 * P.scr_w + x0, cp * w01);
+    sink.add(y1 * P.scr_w + x1, cp * w11);
+}
+
+// One lighttrace bounce (trace_photon body, lighttrace.rs:133-151).
+// Returns 0 = continue, 1 = landed (splatted), 2 = lost.
+template <class Sink>
+__device__ inline int lighttrace_bounce(const SceneData &S, const float4 *tbl, const PhotonParams &P,
+                                        PhotonState &st, Sink &sink) {
+    Hit h;
+    int mi = scene_hit(S, st.ro, st.rd, h);
+    float ss, rr, stt;
+    bool on = screen_coords(P, st.ro, st.rd, ss, rr, stt);
+    float scene_t = mi >= 0 ? h.t : INFINITY;
+    float screen_t = on ? stt : INFINITY;
+    if (screen_t < scene_t) {
+        screen_splat(P, tbl, ss, rr, st.lambda, st.power, sink);
+        return 1;
+    }
+    if (mi < 0) return 2;
+    photon_scatter(S, P, mi, h, st);
+    return 0;
+}
+
+// Point / legacy estimator deposits for one segment (volume.rs:130-150, 51-64).
+template <bool LEGACY, class Sink>
+__device__ inline void volume_point_segment(const PhotonParams &P, const float4 *tbl,
+                                            const PhotonState &st, f3 cmf, float seg_len,
+                                            uint32_t &rng, Sink &sink) {
+    const f3 cam_o = ld3(P.proj.o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float dist = next_f32(rng) * seg_len;
+        f3 p = st.ro + st.rd * dist;
+        float s, t;
+        if (!project(P.proj, p, s, t)) continue;
+        uint32_t idx = pix_index(s, t, P.w, P.h);
+        if (LEGACY) {
+            const float inv4pi = 1.0f / 12.566371f;
+            float contrib = st.power * P.vp.sigma_s * seg_len / 4.0f * inv4pi;
+            sink.add(idx, cmf * contrib);
+        } else {
+            f3 to = cam_o - p;
+            float d = fmaxf(length(to), 1e-3f);
+            float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
+            if (d < zb) {
+                float cos_theta = dot(st.rd, to / d);
+                float phase = phase_hg(P.vp.g, cos_theta);
+                float trans = expf(-P.vp.sigma_t * d);
+                float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
+                sink.add(idx, cmf * contrib);
+            }
+        }
+    }
+}
+
+// Beam estimator deposits for one segment (volume.rs:263-311, 178-218).
+template <class Sink>
+__device__ inline void splat_transverse(const PhotonParams &P, float s, float t, f3 color,
+                                        float radius_px, Sink &sink) {
+    const uint32_t W = P.w, H = P.h;
+    float cx = s * (float)W - 0.5f;
+    float cy = (1.0f - t) * (float)H - 0.5f;
+    float r = fmaxf(radius_px, 0.5f);
+    long x0 = (long)fmaxf(floorf(cx - r), 0.0f);
+    long x1 = (long)ceilf(cx + r);
+    x1 = x1 < 0 ? 0 : (x1 > (long)W - 1 ? (long)W - 1 : x1);
+    long y0 = (long)fmaxf(floorf(cy - r), 0.0f);
+    long y1 = (long)ceilf(cy + r);
+    y1 = y1 < 0 ? 0 : (y1 > (long)H - 1 ? (long)H - 1 : y1);
+    float wsum = 0.0f;
+    for (long py = y0; py <= y1; ++py)
+        for (long px = x0; px <= x1; ++px) {
+            float dx = (float)px - cx, dy = (float)py - cy;
+            wsum += fmaxf(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f);
+        }
+    if (wsum <= 0.0f) {
+        long px = (long)roundf(cx), py = (long)roundf(cy);
+        px = px < 0 ? 0 : (px > (long)W - 1 ? (long)W - 1 : px);
+        py = py < 0 ? 0 : (py > (long)H - 1 ? (long)H - 1 : py);
+        sink.add((uint32_t)(py * (long)W + px), color);
+        return;
+    }
+    float inv = 1.0f / wsum;
+    for (long py = y0; py <= y1; ++py)
+        for (long px = x0; px <= x1; ++px) {
+            float dx = (float)px - cx, dy = (float)py - cy;
+            float w = fmaxf(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f) * inv;
+            if (w > 0.0f) sink.add((uint32_t)(py * (long)W + px), color * w);
+        }
+}
+
+template <class Sink>
+__device__ inline void volume_beam_segment(const PhotonParams &P, const PhotonState &st, f3 cmf,
+                                           float seg_len, Sink &sink) {
+    const f3 cam_o = ld3(P.proj.o);
+    f3 p_end = st.ro + st.rd * seg_len;
+    float s0, t0, s1, t1;
+    uint32_t m;
+    if (project(P.proj, st.ro, s0, t0) && project(P.proj, p_end, s1, t1)) {
+        float dx = (s1 - s0) * (float)P.w;
+        float dy = (t1 - t0) * (float)P.h;
+        float mf = ceilf(sqrtf(dx * dx + dy * dy) * 2.0f);
+        uint64_t mm = (uint64_t)mf + 1u;
+        m = (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+    } else {
+        uint64_t mm = (uint64_t)ceilf(seg_len * 80.0f);
+        m = (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+    }
+    float ds = seg_len / (float)m;
+    for (uint32_t k = 0; k < m; ++k) {
+        float dist = ((float)k + 0.5f) * ds;
+        f3 p = st.ro + st.rd * dist;
+        float s, t;
+        if (!project(P.proj, p, s, t)) continue;
+        uint32_t idx = pix_index(s, t, P.w, P.h);
+        f3 to = cam_o - p;
+        float d = fmaxf(length(to), 1e-3f);
+        float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
+        if (!(d < zb)) continue;
+        f3 view = to / d;
+        float cos_theta = dot(st.rd, view);
+        float phase = phase_hg(P.vp.g, cos_theta);
+        float trans = expf(-P.vp.sigma_t * d);
+        float weight = st.power * P.vp.sigma_s * phase * trans / (d * d) * ds;
+        f3 color = cmf * weight;
+        f3 perp = cross(view, mk(0.f, 1.f, 0.f));
+        if (dot(perp, perp) < 1e-8f) perp = cross(view, mk(1.f, 0.f, 0.f));
+        perp = normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
+        float s2, t2, radius_px;
+        if (project(P.proj, p + perp, s2, t2)) {
+            float dpx = (s2 - s) * (float)P.w;
+            float dpy = (t2 - t) * (float)P.h;
+            radius_px = sqrtf(dpx * dpx + dpy * dpy);
+            radius_px = radius_px < 1.5f ? 1.5f : (radius_px > 24.0f ? 24.0f : radius_px);
+        } else {
+            radius_px = 1.0f;
+        }
+        splat_transverse(P, s, t, color, radius_px, sink);
+    }
+}
+
+// One volume bounce: segment deposits, then scatter.  Returns false when
+// the photon has escaped (its last segment is still deposited).
+template <uint32_t EST, class Sink>
+__device__ inline bool volume_bounce(const SceneData &S, const float4 *tbl, const PhotonParams &P,
+                                     PhotonState &st, Sink &sink) {
+    Hit h;
+    int mi = scene_hit(S, st.ro, st.rd, h);
+    float seg_len = mi >= 0 ? h.t : P.vp.max_dist;
+    f3 cmf = cmf_lookup(tbl, st.lambda);
+    if (EST == SPX_EST_POINT) volume_point_segment<false>(P, tbl, st, cmf, seg_len, st.rng, sink);
+    else if (EST == SPX_EST_LEGACY) volume_point_segment<true>(P, tbl, st, cmf, seg_len, st.rng, sink);
+    else volume_beam_segment(P, st, cmf, seg_len, sink);
+    if (mi < 0) return false;
+    photon_scatter(S, P, mi, h, st);
+    return true;
+}
+
+// Sinks ---------------------------------------------------------------------
+struct GlobalSink {
+    float *film;
+    __device__ inline void add(uint32_t idx, f3 v) { film_add(film, idx, v); }
+};
+
+struct NullSink {
+    __device__ inline void add(uint32_t, f3) {}
+};
+
+}  // namespace spx

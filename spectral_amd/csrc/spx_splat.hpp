@@ -1,0 +1,90 @@
+// spx_splat.hpp -- wave / block level film-splat aggregation in LDS.
+//
+// A block keeps an open-addressed hash of (pixel -> XYZ partial sum) in LDS.
+// Deposits that land on the same pixel within the block are combined with
+// LDS fp32 atomics (ds_add_f32); the table is drained to the film with
+// global fp32 atomics (global_atomic_add_f32) when it is flushed.  A deposit
+// that finds no free slot within kProbes goes straight to the film, so the
+// result never depends on the table size -- only the number of global
+// atomics does.  Summation order changes float rounding only.
+#pragma once
+
+#include "spx_device.hpp"
+
+namespace spx {
+
+constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
+constexpr int kProbes = 8;
+
+template <int SLOTS>
+struct LdsSplat {
+    static_assert((SLOTS & (SLOTS - 1)) == 0, "power of two");
+    uint32_t *keys;  // [SLOTS]
+    float *vals;     // [3*SLOTS]
+    uint32_t *fill;  // occupied-slot counter
+    float *film;
+
+    __device__ inline void init() {
+        for (uint32_t k = threadIdx.x; k < (uint32_t)SLOTS; k += blockDim.x) {
+            keys[k] = kEmptyKey;
+            vals[3 * k] = vals[3 * k + 1] = vals[3 * k + 2] = 0.0f;
+        }
+        if (threadIdx.x == 0) *fill = 0;
+    }
+
+    __device__ inline void lds_add(uint32_t slot, f3 v) {
+        __hip_atomic_fetch_add(vals + 3 * slot + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(vals + 3 * slot + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(vals + 3 * slot + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+
+    __device__ inline void add(uint32_t idx, f3 v) {
+        uint32_t h = (idx * 2654435761u) & (uint32_t)(SLOTS - 1);
+        for (int p = 0; p < kProbes; ++p) {
+            uint32_t slot = (h + (uint32_t)p) & (uint32_t)(SLOTS - 1);
+            uint32_t k = keys[slot];
+            if (k == idx) {
+                lds_add(slot, v);
+                return;
+            }
+            if (k == kEmptyKey) {
+                uint32_t old = atomicCAS(keys + slot, kEmptyKey, idx);
+                if (old == kEmptyKey) {
+                    atomicAdd(fill, 1u);
+                    lds_add(slot, v);
+                    return;
+                }
+                if (old == idx) {
+                    lds_add(slot, v);
+                    return;
+                }
+            }
+        }
+        film_add(film, idx, v);
+    }
+
+    // Whole block must call (contains barriers).  Drains occupied slots.
+    __device__ inline void flush() {
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < (uint32_t)SLOTS; k += blockDim.x) {
+            uint32_t key = keys[k];
+            if (key != kEmptyKey) {
+                film_add(film, key, mk(vals[3 * k], vals[3 * k + 1], vals[3 * k + 2]));
+                keys[k] = kEmptyKey;
+                vals[3 * k] = vals[3 * k + 1] = vals[3 * k + 2] = 0.0f;
+            }
+        }
+        if (threadIdx.x == 0) *fill = 0;
+        __syncthreads();
+    }
+
+    // Whole block must call.  Flushes when more than half the slots are taken.
+    __device__ inline void maybe_flush() {
+        __syncthreads();
+        uint32_t f = *fill;
+        __syncthreads();  // every thread has read the same count
+        if (f > (uint32_t)(SLOTS / 2)) flush();
+    }
+};
+
+}  // namespace spx
