@@ -1,0 +1,245 @@
+"""Benchmark: Mphotons/s + Mpaths/s of the light-transport core on one MI355X.
+
+One step = the mixed stream of 512 uneven jobs (half rainbow camera renders,
+half prism-on-floor volume renders; spectral_amd.jobs.mixed_jobs, seed 0)
+traced by the batched engine in one persistent launch, inputs resident in
+HBM.  value = (paths + photons) per second / 1e6 over exactly K timed steps.
+
+Also reported on the same line: the plain one-job-per-launch form on the same
+stream, the per-configuration throughputs (DSOTM prism 2^20 photons at
+320x180; rainbow 512^2 x 64 spp; full-size prism-on-floor 2^28 photons at
+1920x1080 under D65, A and dispersion off, plus its floor caustic), the
+served stream's p50/p99 job latency, the roofline of the batched kernel and
+the CPU oracle timed on this host's cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mphotons/s + Mpaths/s on 1 MI355X, batched vs plain; p50/p99 job latency"
+UNIT = "Mphotons/s + Mpaths/s"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes (per unit of work): a camera path segment moves a 44 B
+# state in and out (88 B) and a finished path deposits one XYZ (12 B); a
+# photon segment moves a 40 B state in and out (80 B) and each film deposit
+# is 3 fp32 atomics (12 B).
+CAM_SEG_BYTES, CAM_PATH_BYTES, PH_SEG_BYTES, SPLAT_BYTES = 88, 12, 80, 12
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Timer:
+    def __init__(self, ctx, stream):
+        self.ctx, self.stream = ctx, stream
+        self.e0, self.e1 = ctx.event(), ctx.event()
+
+    def run(self, fn, reps=1):
+        self.ctx.sync(self.stream)
+        t0 = time.perf_counter()
+        self.ctx.record(self.e0, self.stream)
+        for _ in range(reps):
+            fn()
+        self.ctx.record(self.e1, self.stream)
+        ms = self.ctx.elapsed_ms(self.e0, self.e1)
+        self.ctx.sync(self.stream)
+        return ms / reps, (time.perf_counter() - t0) * 1e3 / reps
+
+
+def work_model():
+    """Segments per path / photon and deposits per photon, counted by the
+    oracle on samples of the two job classes."""
+    from oracle import oracle as O
+    from spectral_amd import _abi as A
+    from spectral_amd import jobs, scenes
+    cam_spec = jobs.JobSpec(A.JOB_CAMERA, 96, spp=4, seed=1)
+    sc, cam, _ = jobs.job_geometry(cam_spec)
+    _, st = O.camera_accumulate(sc, cam, 96, 96, 4, 0, 1, jobs.job_opts(cam_spec), threads=8)
+    seg_per_path = st.segments / st.paths
+    segs = splats = n = 0
+    for rot, off in ((0.0, 0.0), (20.0, 0.05), (45.0, -0.08)):
+        sp = jobs.JobSpec(A.JOB_VOLUME, 256, n_photons=1 << 13, rotation=rot, offset=off, seed=2)
+        sc, cam, bm = jobs.job_geometry(sp)
+        _, s2 = O.volume(sc, cam, bm, 256, 256, sp.n_photons, 0, A.VolParams(**scenes.HAZE), None, 2,
+                         A.EST_POINT, jobs.job_opts(sp), threads=8)
+        segs, splats, n = segs + s2.segments, splats + s2.splats, n + s2.paths
+    return {"seg_per_path": seg_per_path, "seg_per_photon": segs / n, "splat_per_photon": splats / n}
+
+
+def cpu_baseline(paths_total, photons_total, target_s=12.0):
+    """The oracle on this host: camera paths on all cores (the reference's
+    rayon loop), volume photons single-threaded (volume.rs:118)."""
+    from oracle import oracle as O
+    from spectral_amd import _abi as A
+    from spectral_amd import jobs, scenes
+    cores = min(os.cpu_count() or 1, 16)
+    spec = jobs.JobSpec(A.JOB_CAMERA, 256, spp=2, seed=1)
+    sc, cam, _ = jobs.job_geometry(spec)
+    t0 = time.perf_counter()
+    O.camera_accumulate(sc, cam, 256, 256, 2, 0, 1, jobs.job_opts(spec), threads=cores)
+    t_cam = time.perf_counter() - t0
+    spp = max(1, int(2 * (target_s / 2) / max(t_cam, 1e-3)))
+    spp = min(spp, 64)
+    t0 = time.perf_counter()
+    O.camera_accumulate(sc, cam, 256, 256, spp, 2, 1, jobs.job_opts(spec), threads=cores)
+    t_cam = time.perf_counter() - t0
+    cam_rate = 256 * 256 * spp / t_cam
+    sp = jobs.JobSpec(A.JOB_VOLUME, 512, n_photons=1 << 14, rotation=20.0, seed=2)
+    sc, cam, bm = jobs.job_geometry(sp)
+    vp = A.VolParams(**scenes.HAZE)
+    t0 = time.perf_counter()
+    O.volume(sc, cam, bm, 512, 512, sp.n_photons, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp), threads=1)
+    t1 = time.perf_counter() - t0
+    n = min(1 << 22, max(1 << 14, int(sp.n_photons * (target_s / 2) / max(t1, 1e-3))))
+    t0 = time.perf_counter()
+    O.volume(sc, cam, bm, 512, 512, n, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp), threads=1)
+    t_ph = time.perf_counter() - t0
+    ph_rate = n / t_ph
+    step_s = paths_total / cam_rate + photons_total / ph_rate
+    return {"value": round((paths_total + photons_total) / step_s / 1e6, 3), "unit": UNIT,
+            "cores": cores, "kind": "port",
+            "sample": (f"oracle (C restatement of spectral-core): rainbow 256x256x{spp} spp on {cores} "
+                       f"threads = {cam_rate / 1e6:.3f} Mpaths/s; prism-on-floor volume {n} photons "
+                       f"at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; value = the mixed "
+                       f"step's paths and photons at those rates"),
+            "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4)}
+
+
+def per_config(ctx, timer, quick):
+    """Throughput of each single-job configuration (best engine listed first)."""
+    import spectral_amd as S
+    from spectral_amd import _abi as A
+    from spectral_amd import scenes
+    out = {}
+    st = timer.stream
+    # DSOTM prism, 2^20 photons, 320x180
+    sc = scenes.prism_scene().upload(ctx)
+    bm, cam = scenes.prism_beam(), scenes.prism_camera()
+    film = S.Film(ctx, 320, 180)
+    for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront")):
+        f = lambda: S.render_volume(ctx, sc, cam, bm, 320, 180, 1 << 20, 0.05, 0.05, 0.0, 14.0, None,
+                                    0, film=film, illuminant=A.ILLUM_D65, engine=eng, stream=st)
+        f()
+        ms, _ = timer.run(f, 3)
+        out[f"prism_2^20_320x180_{nm}_mphotons_s"] = round((1 << 20) / ms / 1e3, 1)
+    film.free()
+    # rainbow 512^2 x 64 spp
+    rs, rc = scenes.rainbow_scene().upload(ctx), scenes.rainbow_camera(512, 512)
+    for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront")):
+        tr = S.HipTracer(ctx, rs, rc, 512, 512, A.ILLUM_D65, 1, engine=eng, rr_depth=3)
+        f = lambda: tr.accumulate(64, st)
+        f()
+        ms, _ = timer.run(f, 2)
+        out[f"rainbow_512x512x64_{nm}_mpaths_s"] = round(512 * 512 * 64 / ms / 1e3, 1)
+        tr.film.free()
+    # full-size prism on floor, 1920x1080
+    n = 1 << (24 if quick else 28)
+    fs = scenes.prism_scene(floor=True).upload(ctx)
+    ps = scenes.prism_scene().upload(ctx)
+    fc = scenes.floor_camera(1920, 1080)
+    film = S.Film(ctx, 1920, 1080)
+    scr = scenes.floor_screen(1024, 1024)
+    sfilm = S.Film(ctx, 1024, 1024)
+    for label, ill, disp in (("D65", A.ILLUM_D65, True), ("A", A.ILLUM_A, True), ("disp_off", A.ILLUM_D65, False)):
+        f = lambda: S.render_volume(ctx, fs, fc, bm, 1920, 1080, n, 0.05, 0.05, 0.0, 14.0, None, 0,
+                                    film=film, illuminant=ill, dispersion=disp, engine=A.ENGINE_PLAIN,
+                                    stream=st)
+        ms, _ = timer.run(f, 1)
+        out[f"floor_volume_2^{n.bit_length() - 1}_1920x1080_{label}_mphotons_s"] = round(n / ms / 1e3, 1)
+        g = lambda: S.lighttrace_trace(ctx, ps, scr, bm, n, 0x0F1, film=sfilm, illuminant=ill,
+                                       dispersion=disp, stream=st)
+        ms, _ = timer.run(g, 1)
+        out[f"floor_caustic_2^{n.bit_length() - 1}_{label}_mphotons_s"] = round(n / ms / 1e3, 1)
+    film.free()
+    sfilm.free()
+    return out
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--quick", action="store_true", help="smaller per-config sizes")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the light-transport jobs are independent and this "
+              "build runs no multi-GPU path (see DESIGN.md for how it would shard)", flush=True)
+        return 2
+
+    import spectral_amd as S
+    from spectral_amd import _abi as A
+    from spectral_amd import jobs
+
+    ctx = S.Context(0)
+    stream = ctx.stream()
+    timer = Timer(ctx, stream)
+    specs = jobs.mixed_jobs()
+    paths = sum(s.units for s in specs if s.kind == A.JOB_CAMERA)
+    photons = sum(s.units for s in specs if s.kind == A.JOB_VOLUME)
+    total = paths + photons
+    log(f"mixed stream: {len(specs)} jobs, {paths / 1e9:.3f} G paths, {photons / 1e9:.3f} G photons")
+    br = jobs.BatchRunner(ctx, specs)
+    cfg = A.BatchCfg(0, 0, 0, 0)
+    for _ in range(args.warmup):
+        br.run_batched(stream, cfg)
+    ctx.sync(stream)
+    ms_step, wall_ms = timer.run(lambda: br.run_batched(stream, cfg), args.steps)
+    value = total / ms_step / 1e3
+    log(f"batched: {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
+    ms_plain, _ = timer.run(lambda: br.run_plain(stream), 1)
+    plain_value = total / ms_plain / 1e3
+    log(f"plain one-job-per-launch: {ms_plain:.2f} ms -> {plain_value:.1f}")
+    br.close()
+
+    wm = work_model()
+    alg_bytes = paths * (wm["seg_per_path"] * CAM_SEG_BYTES + CAM_PATH_BYTES) + photons * (
+        wm["seg_per_photon"] * PH_SEG_BYTES + wm["splat_per_photon"] * SPLAT_BYTES)
+    achieved = alg_bytes / (ms_step / 1e3) / 1e9
+    result = {
+        "metric": METRIC, "value": round(value, 2), "unit": UNIT, "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "512 mixed jobs: rainbow camera renders + prism-on-floor volume renders",
+                   "jobs": len(specs), "paths": paths, "photons": photons, "engine": "batched",
+                   "generator_seed": 0},
+        "plain_value": round(plain_value, 2), "batched_over_plain": round(value / plain_value, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_batch", "work_model": {k: round(v, 4) for k, v in wm.items()}},
+    }
+    if not args.no_configs:
+        result["configs"] = per_config(ctx, timer, args.quick)
+        log(json.dumps(result["configs"]))
+    if not args.no_serve:
+        try:
+            from spectral_amd import serve
+            result["serve"] = serve.bench_serve(ctx, quick=args.quick)
+        except ImportError:
+            result["serve"] = None
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(paths, photons)
+    print(json.dumps(result), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -251,7 +251,7 @@ typedef struct {
 } spx_job;
 
 typedef struct {
-    uint32_t block_size;   /* threads per block: 64..1024, multiple of 64 (0 = tuned default) */
+    uint32_t splat_mode;   /* photon deposits: 0 = direct fp32 atomics, 1 = LDS hash first */
     uint32_t blocks;       /* persistent grid size (0 = tuned default) */
     uint32_t chunk_paths;  /* camera work item size in paths (0 = tuned default) */
     uint32_t chunk_photons;/* photon work item size (0 = tuned default) */
@@ -269,6 +269,8 @@ int spx_malloc(spx_ctx *ctx, size_t bytes, void **out);
 int spx_free(spx_ctx *ctx, void *ptr);
 int spx_memcpy_h2d(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
 int spx_memcpy_d2h(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+/* Copies n floats device-to-device with a plain streaming kernel. */
+int spx_copy_f32(spx_ctx *ctx, float *dst, const float *src, size_t n, void *stream);
 /* Fills n floats with value using a kernel (safe inside graph capture). */
 int spx_fill_f32(spx_ctx *ctx, float *dst, float value, size_t n, void *stream);
 int spx_stream_create(spx_ctx *ctx, void **out);

@@ -87,7 +87,7 @@ class Job(C.Structure):
 
 
 class BatchCfg(C.Structure):
-    _fields_ = [("block_size", C.c_uint32), ("blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
+    _fields_ = [("splat_mode", C.c_uint32), ("blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
                 ("chunk_photons", C.c_uint32)]
 
 
@@ -125,6 +125,7 @@ SIGNATURES = {
     "spx_memcpy_h2d": (_i32, [_vp, _vp, _vp, _sz, _vp]),
     "spx_memcpy_d2h": (_i32, [_vp, _vp, _vp, _sz, _vp]),
     "spx_fill_f32": (_i32, [_vp, _vp, _f32, _sz, _vp]),
+    "spx_copy_f32": (_i32, [_vp, _vp, _vp, _sz, _vp]),
     "spx_stream_create": (_i32, [_vp, _P(_vp)]),
     "spx_stream_destroy": (_i32, [_vp, _vp]),
     "spx_stream_sync": (_i32, [_vp, _vp]),

@@ -1,0 +1,165 @@
+// k_batch.hip -- the batched engine: many uneven camera / volume / lighttrace
+// jobs in ONE persistent launch fed from a device job table.
+//
+//  * Work items are (job, range) chunks: camera items are pixel ranges with
+//    all their spp samples, photon items are photon-index ranges.  The host
+//    sorts items by cost (longest first) and deals them to 8 queues, one per
+//    XCD; block b starts on queue b % 8 (blocks are dispatched round-robin
+//    over the XCDs), so a job's film stays warm in one XCD's L2, and steals
+//    from the other queues when its own runs dry.
+//  * Inside an item every lane runs its own path / photon bounce by bounce
+//    and regenerates a new one the moment its current one ends, so the wave
+//    stays full even though path lengths differ (no per-bounce barrier).
+//  * Camera samples are summed per pixel in an LDS accumulator and written
+//    once per item (one owner per pixel: no global atomics).  Photon
+//    deposits go through the LDS splat hash and are drained with fp32 global
+//    atomics at the end of the item.
+#include "spx_batch.hpp"
+#include "spx_paths.hpp"
+#include "spx_splat.hpp"
+
+namespace spx {
+
+constexpr int kBatchSlots = 2048;
+constexpr uint32_t kMaxCamPix = 2048;  // LDS pixel accumulator capacity per item
+
+__device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Camera item: pixels [start, start+count) of job J, all spp samples each.
+__device__ inline void batch_camera_item(const SceneData &S, const float4 *tbl, const CamParams &P,
+                                         uint32_t start, uint32_t count, float *acc) {
+    for (uint32_t k = threadIdx.x; k < 3 * count; k += blockDim.x) acc[k] = 0.0f;
+    __syncthreads();
+    const uint32_t total = count * P.spp;
+    uint32_t f = threadIdx.x;
+    if (f < total) {
+        CamState st;
+        uint32_t lp = f / P.spp;
+        camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f % P.spp, st);
+        uint32_t b = 0;
+        while (true) {
+            bool alive = camera_bounce(S, tbl, P, b, st);
+            ++b;
+            if (alive && b < P.max_depth) continue;
+            f3 c = camera_xyz(tbl, st);
+            __hip_atomic_fetch_add(acc + 3 * lp + 0, c.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc + 3 * lp + 1, c.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            __hip_atomic_fetch_add(acc + 3 * lp + 2, c.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            f += blockDim.x;
+            if (f >= total) break;
+            lp = f / P.spp;
+            camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f % P.spp, st);
+            b = 0;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) {
+        uint32_t i = start + k;
+        P.film[3 * i + 0] += acc[3 * k + 0];
+        P.film[3 * i + 1] += acc[3 * k + 1];
+        P.film[3 * i + 2] += acc[3 * k + 2];
+    }
+}
+
+template <class Sink>
+__device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, const PhotonParams &P,
+                                         uint32_t kind, uint32_t start, uint32_t count, Sink &sink) {
+    uint32_t f = threadIdx.x;
+    if (f >= count) return;
+    PhotonState st;
+    photon_begin(P, tbl, P.photon_base + start + f, st);
+    uint32_t b = 0;
+    while (true) {
+        bool alive;
+        if (kind == SPX_JOB_LIGHTTRACE) alive = lighttrace_bounce(S, tbl, P, st, sink) == 0;
+        else if (P.estimator == SPX_EST_LEGACY) alive = volume_bounce<SPX_EST_LEGACY>(S, tbl, P, st, sink);
+        else alive = volume_bounce<SPX_EST_POINT>(S, tbl, P, st, sink);
+        ++b;
+        if (alive && b < P.max_depth) continue;
+        f += blockDim.x;
+        if (f >= count) break;
+        photon_begin(P, tbl, P.photon_base + start + f, st);
+        b = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs,
+                                               const BatchItem *__restrict__ items,
+                                               BatchQueues *q, uint32_t splat_mode) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    __shared__ uint32_t sk[kBatchSlots];
+    __shared__ float sv[3 * kBatchSlots];
+    __shared__ uint32_t sfill;
+    __shared__ uint32_t s_item;
+    __shared__ const SceneData *s_scene;
+    __shared__ const float4 *s_tbl;
+    float *acc = sv;  // camera items reuse the splat value array as the pixel accumulator
+    LdsSplat<kBatchSlots> hash{sk, sv, &sfill, nullptr};
+    if (threadIdx.x == 0) {
+        s_scene = nullptr;
+        s_tbl = nullptr;
+    }
+    hash.init();
+    uint32_t home = blockIdx.x % kXcdQueues;
+    uint32_t qi = 0;  // queues tried so far (home first, then the others)
+    while (true) {
+        if (threadIdx.x == 0) {
+            uint32_t it = 0xFFFFFFFFu;
+            while (qi < kXcdQueues) {
+                uint32_t qq = (home + qi) % kXcdQueues;
+                uint32_t k = atomicAdd(&q->next[qq], 1u);
+                if (k < q->size[qq]) {
+                    it = q->begin[qq] + k;
+                    break;
+                }
+                ++qi;
+            }
+            s_item = it;
+        }
+        __syncthreads();
+        const uint32_t it = uniform_u32(s_item);
+        if (it == 0xFFFFFFFFu) break;
+        const BatchItem item = items[it];
+        const uint32_t j = uniform_u32(item.job);
+        const uint32_t kind = uniform_u32(jobs[j].kind);
+        const SceneData *scene = kind == SPX_JOB_CAMERA ? jobs[j].cam.scene : jobs[j].ph.scene;
+        const float4 *jt = kind == SPX_JOB_CAMERA ? jobs[j].cam.tbl : jobs[j].ph.tbl;
+        if (scene != s_scene || jt != s_tbl) {  // restage only when the job's scene changes
+            __syncthreads();
+            stage_scene(S, scene, tbl, jt);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                s_scene = scene;
+                s_tbl = jt;
+            }
+        }
+        __syncthreads();
+        if (kind == SPX_JOB_CAMERA) {
+            const CamParams P = jobs[j].cam;
+            batch_camera_item(S, tbl, P, item.start, item.count, acc);
+            __syncthreads();
+            // restore the hash's empty value array for the next photon item
+            for (uint32_t k = threadIdx.x; k < 3 * item.count; k += blockDim.x) sv[k] = 0.0f;
+        } else {
+            const PhotonParams P = jobs[j].ph;
+            if (splat_mode == 0) {
+                GlobalSink g{P.film};
+                batch_photon_item(S, tbl, P, kind, item.start, item.count, g);
+            } else {
+                hash.film = P.film;
+                batch_photon_item(S, tbl, P, kind, item.start, item.count, hash);
+                hash.flush();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueues *q, uint32_t blocks,
+                        uint32_t splat_mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(256), 0, s, jobs, items, q, splat_mode);
+    return hipGetLastError();
+}
+
+}  // namespace spx

@@ -233,4 +233,18 @@ hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s) {
     return hipGetLastError();
 }
 
+__global__ void k_copy(float4 *dst, const float4 *src, size_t n4) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(float *dst, const float *src, size_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if ((n & 3) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15)) return hipErrorInvalidValue;
+    size_t n4 = n / 4, blocks = (n4 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(k_copy, dim3((uint32_t)blocks), dim3(256), 0, s, (float4 *)dst, (const float4 *)src, n4);
+    return hipGetLastError();
+}
+
 }  // namespace spx

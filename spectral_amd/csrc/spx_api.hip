@@ -7,6 +7,10 @@
 
 #include <string>
 
+#include <algorithm>
+#include <vector>
+
+#include "spx_batch.hpp"
 #include "spx_internal.hpp"
 #include "_gen_cie.inc"
 
@@ -165,6 +169,9 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->batch_dev) hipFree(ctx->batch_dev);
+    if (ctx->batch_host) hipHostFree(ctx->batch_host);
+    if (ctx->batch_done) hipEventDestroy(ctx->batch_done);
     if (ctx->tbl_dev) hipFree(ctx->tbl_dev);
     delete ctx;
 }
@@ -463,6 +470,177 @@ int spx_glass_n(spx_ctx *ctx, uint32_t glass, const float *lambda_host, uint32_t
     return check_hip(ctx, e, "spx_glass_n");
 }
 
+// ---- batched jobs ------------------------------------------------------------
+static int build_batch_job(spx_ctx *ctx, const spx_job &jb, BatchJob &out, uint32_t idx) {
+    char who[64];
+    snprintf(who, sizeof who, "spx_batch_run job %u", idx);
+    memset(&out, 0, sizeof out);
+    out.kind = jb.kind;
+    if (!jb.scene || jb.scene->ctx != ctx) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad scene");
+    int rc;
+    if (jb.kind == SPX_JOB_CAMERA) {
+        if ((rc = check_film(ctx, jb.film, jb.width, jb.height, who))) return rc;
+        if (!valid_opts(&jb.opts, true)) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad options");
+        if (!finite3(jb.camera.origin) || !finite3(jb.camera.lower_left) || !finite3(jb.camera.horizontal) ||
+            !finite3(jb.camera.vertical))
+            return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite camera");
+        CamParams &p = out.cam;
+        p.cam = jb.camera;
+        p.w = jb.width; p.h = jb.height; p.spp = jb.spp; p.sample_base = jb.sample_base; p.seed = jb.seed;
+        p.illuminant = jb.opts.illuminant; p.dispersion = jb.opts.dispersion;
+        p.max_depth = jb.opts.max_depth; p.rr_depth = jb.opts.rr_depth;
+        p.scene = jb.scene->dev;
+        p.tbl = table_for(ctx, jb.opts.illuminant);
+        p.film = jb.film;
+        return SPX_OK;
+    }
+    if (jb.kind != SPX_JOB_VOLUME && jb.kind != SPX_JOB_LIGHTTRACE)
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": unknown kind");
+    PhotonParams &p = out.ph;
+    if ((rc = photon_params(ctx, jb.scene, &jb.beam, &jb.opts, jb.n_photons, jb.photon_base, jb.seed, p, who)))
+        return rc;
+    if (jb.kind == SPX_JOB_LIGHTTRACE) {
+        if ((rc = check_film(ctx, jb.film, jb.screen.width, jb.screen.height, who))) return rc;
+        set_screen(p, jb.screen);
+    } else {
+        if ((rc = check_film(ctx, jb.film, jb.width, jb.height, who))) return rc;
+        if (jb.estimator != SPX_EST_POINT && jb.estimator != SPX_EST_LEGACY)
+            return set_error(ctx, SPX_EINVAL, std::string(who) + ": batches run the point or legacy estimator");
+        const spx_vol_params &vp = jb.vol;
+        if (!isfinite(vp.sigma_s) || !isfinite(vp.sigma_t) || !isfinite(vp.g) || !(vp.max_dist > 0.0f) ||
+            !isfinite(vp.max_dist))
+            return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad volume parameters");
+        if (!finite3(jb.camera.origin) || !finite3(jb.camera.lower_left) || !finite3(jb.camera.horizontal) ||
+            !finite3(jb.camera.vertical))
+            return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite camera");
+        make_proj(jb.camera, p.proj);
+        p.vp = vp;
+        p.w = jb.width;
+        p.h = jb.height;
+        p.estimator = jb.estimator;
+        p.zbuf = jb.zbuf;
+    }
+    p.film = jb.film;
+    return SPX_OK;
+}
+
+static int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
+    if (ctx->batch_pending) {  // the previous staging copy may still read the pinned buffer
+        hipError_t e = hipEventSynchronize(ctx->batch_done);
+        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run wait");
+        ctx->batch_pending = false;
+    }
+    if (!ctx->batch_done) {
+        hipError_t e = hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming);
+        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run event");
+    }
+    if (ctx->batch_host_bytes < bytes) {
+        if (ctx->batch_host) hipHostFree(ctx->batch_host);
+        ctx->batch_host = nullptr;
+        ctx->batch_host_bytes = 0;
+        hipError_t e = hipHostMalloc(&ctx->batch_host, bytes * 2, hipHostMallocDefault);
+        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run pinned staging");
+        ctx->batch_host_bytes = bytes * 2;
+    }
+    if (ctx->batch_dev_bytes < bytes) {
+        hipError_t e = hipStreamSynchronize(s);  // earlier launches may still read the old table
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run sync");
+        if (ctx->batch_dev) hipFree(ctx->batch_dev);
+        ctx->batch_dev = nullptr;
+        ctx->batch_dev_bytes = 0;
+        e = hipMalloc(&ctx->batch_dev, bytes * 2);
+        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run table");
+        ctx->batch_dev_bytes = bytes * 2;
+    }
+    return SPX_OK;
+}
+
+int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    if (!jobs && n_jobs) return set_error(ctx, SPX_EINVAL, "spx_batch_run: null job table");
+    if (n_jobs == 0) return SPX_OK;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t chunk_paths = cfg && cfg->chunk_paths ? cfg->chunk_paths : 16384u;
+    uint32_t chunk_photons = cfg && cfg->chunk_photons ? cfg->chunk_photons : 8192u;
+    uint32_t blocks = cfg && cfg->blocks ? cfg->blocks : (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
+    uint32_t splat_mode = cfg ? cfg->splat_mode : 0u;
+    if (splat_mode > 1) return set_error(ctx, SPX_EINVAL, "spx_batch_run: unknown splat mode");
+    std::vector<BatchJob> table(n_jobs);
+    std::vector<BatchItem> items;
+    std::vector<double> job_cost(n_jobs, 0.0);
+    std::vector<std::pair<const float *, uint32_t>> films;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        int rc = build_batch_job(ctx, jobs[j], table[j], j);
+        if (rc) return rc;
+        films.push_back({jobs[j].film, jobs[j].kind});
+        const spx_job &jb = jobs[j];
+        if (jb.kind == SPX_JOB_CAMERA) {
+            if (jb.spp == 0) continue;
+            uint32_t npix = jb.width * jb.height;
+            uint32_t per = chunk_paths / jb.spp;
+            per = per < 1 ? 1 : (per > 2048 ? 2048 : per);
+            for (uint32_t p0 = 0; p0 < npix; p0 += per) {
+                uint32_t c = npix - p0 < per ? npix - p0 : per;
+                uint64_t cost = (uint64_t)c * jb.spp;
+                items.push_back({j, p0, c, (uint32_t)std::min<uint64_t>(cost, 0xFFFFFFFFu)});
+                job_cost[j] += (double)cost;
+            }
+        } else {
+            for (uint32_t p0 = 0; p0 < jb.n_photons; p0 += chunk_photons) {
+                uint32_t c = jb.n_photons - p0 < chunk_photons ? jb.n_photons - p0 : chunk_photons;
+                items.push_back({j, p0, c, 2u * c});
+                job_cost[j] += 2.0 * c;
+            }
+        }
+    }
+    // Camera films are written without atomics: no two jobs may share one.
+    std::sort(films.begin(), films.end());
+    for (size_t k = 1; k < films.size(); ++k)
+        if (films[k].first == films[k - 1].first && (films[k].second == SPX_JOB_CAMERA || films[k - 1].second == SPX_JOB_CAMERA))
+            return set_error(ctx, SPX_EINVAL, "spx_batch_run: a camera job shares its film with another job");
+    if (items.empty()) return SPX_OK;
+    // Jobs to XCD queues, largest first onto the least-loaded queue.
+    std::vector<uint32_t> order(n_jobs);
+    for (uint32_t j = 0; j < n_jobs; ++j) order[j] = j;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return job_cost[a] > job_cost[b]; });
+    std::vector<uint32_t> job_queue(n_jobs, 0);
+    double load[kXcdQueues] = {0};
+    for (uint32_t j : order) {
+        uint32_t best = 0;
+        for (uint32_t q = 1; q < kXcdQueues; ++q)
+            if (load[q] < load[best]) best = q;
+        job_queue[j] = best;
+        load[best] += job_cost[j];
+    }
+    std::stable_sort(items.begin(), items.end(), [&](const BatchItem &a, const BatchItem &b) {
+        uint32_t qa = job_queue[a.job], qb = job_queue[b.job];
+        if (qa != qb) return qa < qb;
+        return a.cost > b.cost;
+    });
+    BatchQueues qh;
+    memset(&qh, 0, sizeof qh);
+    for (const BatchItem &it : items) qh.size[job_queue[it.job]]++;
+    for (uint32_t q = 1; q < kXcdQueues; ++q) qh.begin[q] = qh.begin[q - 1] + qh.size[q - 1];
+    size_t off_items = (sizeof(BatchQueues) + 255) & ~(size_t)255;
+    size_t off_jobs = (off_items + items.size() * sizeof(BatchItem) + 255) & ~(size_t)255;
+    size_t bytes = off_jobs + table.size() * sizeof(BatchJob);
+    int rc = batch_stage(ctx, bytes, s);
+    if (rc) return rc;
+    char *h = (char *)ctx->batch_host;
+    memcpy(h, &qh, sizeof qh);
+    memcpy(h + off_items, items.data(), items.size() * sizeof(BatchItem));
+    memcpy(h + off_jobs, table.data(), table.size() * sizeof(BatchJob));
+    char *d = (char *)ctx->batch_dev;
+    hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run upload");
+    ctx->batch_pending = true;
+    e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items), (BatchQueues *)d, blocks,
+                     splat_mode, s);
+    return check_hip(ctx, e, "spx_batch_run launch");
+}
+
 // ---- memory / streams / events ----------------------------------------------
 int spx_malloc(spx_ctx *ctx, size_t bytes, void **out) {
     if (!ctx || !out || bytes == 0) return set_error(ctx, SPX_EINVAL, "spx_malloc: bad arguments");
@@ -491,6 +669,11 @@ int spx_memcpy_d2h(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void 
 int spx_fill_f32(spx_ctx *ctx, float *dst, float value, size_t n, void *stream) {
     if (!ctx || (!dst && n)) return set_error(ctx, SPX_EINVAL, "spx_fill_f32: null");
     return check_hip(ctx, launch_fill(dst, value, n, (hipStream_t)stream), "spx_fill_f32");
+}
+int spx_copy_f32(spx_ctx *ctx, float *dst, const float *src, size_t n, void *stream) {
+    if (!ctx || (n && (!dst || !src))) return set_error(ctx, SPX_EINVAL, "spx_copy_f32: null");
+    if (n & 3) return set_error(ctx, SPX_EINVAL, "spx_copy_f32: n must be a multiple of 4");
+    return check_hip(ctx, launch_copy(dst, src, n, (hipStream_t)stream), "spx_copy_f32");
 }
 int spx_stream_create(spx_ctx *ctx, void **out) {
     if (!ctx || !out) return SPX_EINVAL;

@@ -51,9 +51,15 @@ struct spx_ctx {
     float tbl_host[2][spx::kTblN][4];
     double ill_mean[2];
     std::string last_error;
-    void *scratch = nullptr;   // grow-only work buffer for queues / job tables
+    void *scratch = nullptr;   // grow-only work buffer for the wavefront queues
     size_t scratch_bytes = 0;
     hipStream_t scratch_stream = nullptr;
+    void *batch_dev = nullptr;    // grow-only device job table + items + queues
+    size_t batch_dev_bytes = 0;
+    void *batch_host = nullptr;   // pinned staging for the job table
+    size_t batch_host_bytes = 0;
+    hipEvent_t batch_done = nullptr;  // recorded after the staging copy
+    bool batch_pending = false;
 };
 
 struct spx_scene {
@@ -86,6 +92,7 @@ hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float
 hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out, hipStream_t s);
 hipError_t launch_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s);
+hipError_t launch_copy(float *dst, const float *src, size_t n, hipStream_t s);
 
 // Wavefront engine (per-bounce SoA queues).
 size_t camera_wavefront_scratch(uint32_t chunk);

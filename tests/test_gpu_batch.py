@@ -1,0 +1,104 @@
+"""The batched engine: one persistent launch over a device job table gives
+each job the same film as running it alone (one launch per job), and the
+films agree with the CPU oracle."""
+import numpy as np
+import pytest
+
+from spectral_amd import _abi as A
+from spectral_amd import jobs
+
+from _metrics import rel_l1_per_channel
+
+pytestmark = pytest.mark.gpu
+
+ALONE_TOL = 1e-4  # alone vs in the mix: float summation order only
+FILM_TOL = 5e-3   # against the oracle
+
+
+def _small_mix(n=16):
+    return jobs.mixed_jobs(n=n, seed=3, max_side=160, max_photons_log2=16, max_spp=12)
+
+
+@pytest.mark.parametrize("splat_mode", [0, 1])
+def test_batch_equals_alone(ctx, splat_mode):
+    specs = _small_mix()
+    br = jobs.BatchRunner(ctx, specs)
+    try:
+        br.clear()
+        br.run_batched(cfg=A.BatchCfg(splat_mode, 0, 4096, 2048))
+        mix = [f.image() for f in br.films]
+        br.clear()
+        br.run_plain()
+        alone = [f.image() for f in br.films]
+        for j, (a, b) in enumerate(zip(mix, alone)):
+            assert a.sum() > 0, f"job {j} empty"
+            err = rel_l1_per_channel(a, b)
+            assert (err < ALONE_TOL).all(), f"job {j} ({specs[j]}): {err}"
+    finally:
+        br.close()
+
+
+def test_batch_matches_oracle(ctx, oracle):
+    specs = _small_mix(8)
+    br = jobs.BatchRunner(ctx, specs)
+    try:
+        br.clear()
+        br.run_batched()
+        for j, sp in enumerate(specs):
+            g = br.films[j].image()
+            sc, cam, bm = jobs.job_geometry(sp)
+            o = jobs.job_opts(sp)
+            if sp.kind == A.JOB_CAMERA:
+                c, _ = oracle.camera_accumulate(sc, cam, sp.size, sp.size, sp.spp, 0, sp.seed, o,
+                                                threads=8)
+            else:
+                vp = A.VolParams(**jobs.scenes.HAZE)
+                c, _ = oracle.volume(sc, cam, bm, sp.size, sp.size, sp.n_photons, 0, vp, None,
+                                     sp.seed, A.EST_POINT, o, threads=8)
+            err = rel_l1_per_channel(g, c)
+            print(j, sp.kind, sp.size, sp.units, err)
+            assert (err < FILM_TOL).all(), f"job {j}: {err}"
+    finally:
+        br.close()
+
+
+def test_batch_progressive_bases(ctx):
+    # RNG counters continue across batched frames: two frames with bases
+    # (0, spp) equal one job with 2*spp samples.
+    specs = [jobs.JobSpec(A.JOB_CAMERA, 96, spp=6, seed=4),
+             jobs.JobSpec(A.JOB_VOLUME, 96, n_photons=20000, seed=5, rotation=10.0)]
+    two = jobs.BatchRunner(ctx, specs)
+    one = jobs.BatchRunner(ctx, [jobs.JobSpec(A.JOB_CAMERA, 96, spp=12, seed=4),
+                                 jobs.JobSpec(A.JOB_VOLUME, 96, n_photons=40000, seed=5,
+                                              rotation=10.0)])
+    try:
+        two.clear()
+        two.run_batched()
+        two.set_bases([6, 0], [0, 20000])
+        two.run_batched()
+        one.clear()
+        one.run_batched()
+        for a, b in zip(two.films, one.films):
+            assert (rel_l1_per_channel(a.image(), b.image()) < ALONE_TOL).all()
+    finally:
+        two.close()
+        one.close()
+
+
+def test_batch_rejects_bad_jobs(ctx):
+    specs = [jobs.JobSpec(A.JOB_CAMERA, 32, spp=2, seed=1)]
+    br = jobs.BatchRunner(ctx, specs)
+    try:
+        br.table[0].width = 0
+        with pytest.raises(A.SpxError):
+            br.run_batched()
+        br.table[0].width = 32
+        br.table[0].kind = 9
+        with pytest.raises(A.SpxError):
+            br.run_batched()
+        br.table[0].kind = A.JOB_CAMERA
+        br.clear()
+        br.run_batched()
+        assert br.films[0].image().sum() > 0
+    finally:
+        br.close()
