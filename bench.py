@@ -196,7 +196,8 @@ def main() -> int:
     total = paths + photons
     log(f"mixed stream: {len(specs)} jobs, {paths / 1e9:.3f} G paths, {photons / 1e9:.3f} G photons")
     br = jobs.BatchRunner(ctx, specs)
-    cfg = A.BatchCfg(0, 0, 0, 0)
+    t = ctx.tuning()
+    cfg = A.BatchCfg(t.splat_mode, 0, 0, 0)
     for _ in range(args.warmup):
         br.run_batched(stream, cfg)
     ctx.sync(stream)

@@ -154,6 +154,15 @@ typedef struct {
 typedef struct spx_ctx spx_ctx;
 typedef struct spx_scene spx_scene;
 
+/* Per-context tuning of the wavefront and batched engines (the plain engine
+ * ignores it).  spx_autotune picks these values. */
+typedef struct {
+    uint32_t max_replicas;   /* photon film copies per job (1 = deposit straight into the film) */
+    uint32_t replica_mib;    /* per-job budget for the copies, MiB */
+    uint32_t splat_mode;     /* 0 = direct fp32 atomics, 1 = LDS hash first */
+    uint32_t chunk_log2;     /* wavefront chunk: 2^chunk_log2 paths / photons */
+} spx_tuning;
+
 /* ---- context --------------------------------------------------------------- */
 /* GpuContext::new (spectral-gpu/src/lib.rs:21-62).  Uploads the CIE 1931 CMF
  * and D65/A tables (cie.rs:6-8, data.rs:56-66). */
@@ -162,6 +171,8 @@ void spx_ctx_destroy(spx_ctx *ctx);
 const char *spx_last_error(const spx_ctx *ctx);
 /* Number of compute units of the context's device. */
 int spx_ctx_num_cus(const spx_ctx *ctx);
+int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t);
+int spx_ctx_get_tuning(const spx_ctx *ctx, spx_tuning *t);
 
 /* Camera::look_at (camera.rs:15-27), evaluated on the host.  Needs no device. */
 int spx_camera_look_at(const float origin[3], const float target[3], const float up[3],
@@ -269,6 +280,10 @@ int spx_malloc(spx_ctx *ctx, size_t bytes, void **out);
 int spx_free(spx_ctx *ctx, void *ptr);
 int spx_memcpy_h2d(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
 int spx_memcpy_d2h(spx_ctx *ctx, void *dst, const void *src, size_t bytes, void *stream);
+/* fp32 atomic-add throughput probe: threads x ops hardware atomics spread by
+ * a hash over n_slots floats of buf (device).  Asynchronous on stream. */
+int spx_atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads,
+                     void *stream);
 /* Copies n floats device-to-device with a plain streaming kernel. */
 int spx_copy_f32(spx_ctx *ctx, float *dst, const float *src, size_t n, void *stream);
 /* Fills n floats with value using a kernel (safe inside graph capture). */

@@ -86,6 +86,11 @@ class Job(C.Structure):
                 ("film", C.c_void_p), ("zbuf", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("max_replicas", C.c_uint32), ("replica_mib", C.c_uint32),
+                ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32)]
+
+
 class BatchCfg(C.Structure):
     _fields_ = [("splat_mode", C.c_uint32), ("blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
                 ("chunk_photons", C.c_uint32)]
@@ -103,6 +108,9 @@ SIGNATURES = {
     "spx_ctx_destroy": (None, [_vp]),
     "spx_last_error": (C.c_char_p, [_vp]),
     "spx_ctx_num_cus": (_i32, [_vp]),
+    "spx_ctx_set_tuning": (_i32, [_vp, _P(Tuning)]),
+    "spx_ctx_get_tuning": (_i32, [_vp, _P(Tuning)]),
+    "spx_atomic_probe": (_i32, [_vp, _vp, _sz, _u32, _u32, _vp]),
     "spx_camera_look_at": (_i32, [_P(_f32), _P(_f32), _P(_f32), _f32, _f32, _P(CameraBasis)]),
     "spx_scene_create": (_i32, [_vp, _P(Env), _P(Prim), _u32, _P(Plane), _u32, _P(Material), _u32,
                                 _P(_vp)]),

@@ -41,6 +41,17 @@ class Context:
     def num_cus(self) -> int:
         return self._lib.spx_ctx_num_cus(self.handle)
 
+    def tuning(self) -> A.Tuning:
+        t = A.Tuning()
+        _check(self, self._lib.spx_ctx_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    def set_tuning(self, **kw) -> None:
+        t = self.tuning()
+        for k, v in kw.items():
+            setattr(t, k, int(v))
+        _check(self, self._lib.spx_ctx_set_tuning(self.handle, C.byref(t)))
+
     def close(self) -> None:
         if self.handle:
             self._lib.spx_ctx_destroy(self.handle)

@@ -144,10 +144,10 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
         } else {
             const PhotonParams P = jobs[j].ph;
             if (splat_mode == 0) {
-                GlobalSink g{P.film};
+                GlobalSink g{deposit_film(P)};
                 batch_photon_item(S, tbl, P, kind, item.start, item.count, g);
             } else {
-                hash.film = P.film;
+                hash.film = deposit_film(P);
                 batch_photon_item(S, tbl, P, kind, item.start, item.count, hash);
                 hash.flush();
             }
@@ -160,6 +160,67 @@ hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueue
                         uint32_t splat_mode, hipStream_t s) {
     hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(256), 0, s, jobs, items, q, splat_mode);
     return hipGetLastError();
+}
+
+// Sums R film copies into the caller's film: one block per 1024 floats.
+__global__ __launch_bounds__(256) void k_reduce_replicas(const ReplicaJob *jobs, uint32_t n_jobs) {
+    // binary search the job owning this block
+    uint32_t lo = 0, hi = n_jobs;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (jobs[mid].first_block <= blockIdx.x) lo = mid;
+        else hi = mid;
+    }
+    const ReplicaJob J = jobs[lo];
+    uint32_t base = (blockIdx.x - J.first_block) * 1024u;
+    for (uint32_t k = threadIdx.x; k < 1024u; k += blockDim.x) {
+        uint32_t i = base + k;
+        if (i >= J.n) break;
+        float s = J.dst[i];
+        for (uint32_t r = 0; r < J.reps; ++r) s += J.src[(size_t)r * J.n + i];
+        J.dst[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_one(ReplicaJob J) {
+    uint32_t base = blockIdx.x * 1024u;
+    for (uint32_t k = threadIdx.x; k < 1024u; k += blockDim.x) {
+        uint32_t i = base + k;
+        if (i >= J.n) break;
+        float s = J.dst[i];
+        for (uint32_t r = 0; r < J.reps; ++r) s += J.src[(size_t)r * J.n + i];
+        J.dst[i] = s;
+    }
+}
+
+hipError_t launch_reduce_one(const ReplicaJob &J, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_one, dim3((J.n + 1023) / 1024), dim3(256), 0, s, J);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_replicas(const ReplicaJob *jobs_dev, uint32_t n_jobs, uint32_t total_blocks,
+                                  hipStream_t s) {
+    if (!n_jobs || !total_blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_replicas, dim3(total_blocks), dim3(256), 0, s, jobs_dev, n_jobs);
+    return hipGetLastError();
+}
+
+// fp32 atomic throughput probe: each of `threads` threads adds 1.0 to `ops`
+// addresses drawn by a hash from [0, n_slots).
+__global__ __launch_bounds__(256) void k_atomic_probe(float *buf, uint32_t n_slots, uint32_t ops, uint32_t threads) {
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= threads) return;
+    for (uint32_t k = 0; k < ops; ++k) {
+        uint32_t h = pcg32(t * ops + k);
+        unsafeAtomicAdd(buf + (h % n_slots), 1.0f);
+    }
+}
+
+int atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, hipStream_t s) {
+    (void)ctx;
+    hipLaunchKernelGGL(k_atomic_probe, dim3((threads + 255) / 256), dim3(256), 0, s, buf, (uint32_t)n_slots,
+                       ops, threads);
+    return hipGetLastError() == hipSuccess ? SPX_OK : SPX_EDEVICE;
 }
 
 }  // namespace spx

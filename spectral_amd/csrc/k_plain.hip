@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_volume_plain(PhotonParams P) {
     if (i >= P.n) return;
     PhotonState st;
     photon_begin(P, tbl, P.photon_base + i, st);
-    GlobalSink sink{P.film};
+    GlobalSink sink{deposit_film(P)};
     for (uint32_t b = 0; b < P.max_depth; ++b)
         if (!volume_bounce<EST>(S, tbl, P, st, sink)) break;
 }
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void k_lighttrace_plain(PhotonParams P) {
     if (i >= P.n) return;
     PhotonState st;
     photon_begin(P, tbl, P.photon_base + i, st);
-    GlobalSink sink{P.film};
+    GlobalSink sink{deposit_film(P)};
     for (uint32_t b = 0; b < P.max_depth; ++b)
         if (lighttrace_bounce(S, tbl, P, st, sink) != 0) break;
 }

@@ -239,7 +239,7 @@ __device__ inline bool ph_bounce(const SceneData &S, const float4 *tbl, const Ph
     __shared__ uint32_t sk[kSplatSlots];                 \
     __shared__ float sv[3 * kSplatSlots];                \
     __shared__ uint32_t sfill;                           \
-    LdsSplat<kSplatSlots> sink{sk, sv, &sfill, P.film};
+    LdsSplat<kSplatSlots> sink{sk, sv, &sfill, deposit_film(P)};
 
 template <int MODE>
 __global__ __launch_bounds__(256) void k_ph_first(PhotonParams P, PhWave W, uint32_t start, uint32_t count) {

@@ -62,6 +62,62 @@ void make_proj(const spx_camera &c, Proj &P) {
     P.pad[0] = P.pad[1] = 0.0f;
 }
 
+int ensure_replicas(spx_ctx *ctx, size_t floats, hipStream_t stream) {
+    if (ctx->replica_floats >= floats) return SPX_OK;
+    if (ctx->replica) {
+        hipError_t e = hipDeviceSynchronize();  // earlier launches may still use the copies
+        if (e != hipSuccess) return check_hip(ctx, e, "hipDeviceSynchronize");
+        hipFree(ctx->replica);
+        ctx->replica = nullptr;
+        ctx->replica_floats = 0;
+    }
+    (void)stream;
+    size_t want = floats + floats / 4;
+    hipError_t e = hipMalloc(&ctx->replica, want * sizeof(float));
+    if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(replicas)");
+    ctx->replica_floats = want;
+    return SPX_OK;
+}
+
+uint32_t replica_count(const spx_ctx *ctx, size_t film_floats) {
+    const spx_tuning &t = ctx->tuning;
+    if (t.max_replicas <= 1 || film_floats == 0) return 1;
+    size_t budget = (size_t)t.replica_mib << 20;
+    size_t r = budget / (film_floats * sizeof(float));
+    if (r > t.max_replicas) r = t.max_replicas;
+    return r < 2 ? 1u : (uint32_t)r;
+}
+
+// Runs `launch` with deposits spread over film replicas, then sums them
+// into p.film.  Falls back to the film itself when one copy is chosen.
+template <class F>
+static hipError_t with_replicas(spx_ctx *ctx, PhotonParams &p, size_t film_floats, hipStream_t s, F launch,
+                                int *rc) {
+    *rc = SPX_OK;
+    uint32_t R = replica_count(ctx, film_floats);
+    p.replicas = 1;
+    p.rep_stride = 0;
+    if (R <= 1) return launch(p);
+    if ((*rc = ensure_replicas(ctx, R * film_floats, s))) return hipSuccess;
+    float *film = p.film;
+    hipError_t e = launch_fill(ctx->replica, 0.0f, R * film_floats, s);
+    if (e != hipSuccess) return e;
+    p.film = ctx->replica;
+    p.replicas = R;
+    p.rep_stride = (uint32_t)film_floats;
+    e = launch(p);
+    if (e != hipSuccess) return e;
+    ReplicaJob J;
+    J.dst = film;
+    J.src = ctx->replica;
+    J.n = (uint32_t)film_floats;
+    J.reps = R;
+    J.first_block = 0;
+    J.pad = 0;
+    p.film = film;
+    return launch_reduce_one(J, s);
+}
+
 float illuminant_scale(const spx_ctx *ctx, uint32_t ill) {
     if (ill > SPX_ILLUM_A) return 1.0f;
     return (float)(1.0 / ctx->ill_mean[ill]);
@@ -160,8 +216,33 @@ int spx_ctx_create(int device, spx_ctx **out) {
         delete ctx;
         return e == hipErrorOutOfMemory ? SPX_ENOMEM : SPX_EDEVICE;
     }
+    ctx->tuning.max_replicas = 16;
+    ctx->tuning.replica_mib = 64;
+    ctx->tuning.splat_mode = 1;
+    ctx->tuning.chunk_log2 = 22;
     *out = ctx;
     return SPX_OK;
+}
+
+int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
+    if (!ctx || !t) return SPX_EINVAL;
+    if (t->max_replicas < 1 || t->max_replicas > 256 || t->splat_mode > 1 || t->chunk_log2 < 12 ||
+        t->chunk_log2 > 26 || t->replica_mib > 65536)
+        return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
+    ctx->tuning = *t;
+    return SPX_OK;
+}
+
+int spx_ctx_get_tuning(const spx_ctx *ctx, spx_tuning *t) {
+    if (!ctx || !t) return SPX_EINVAL;
+    *t = ctx->tuning;
+    return SPX_OK;
+}
+
+int spx_atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, void *stream) {
+    if (!ctx || !buf || n_slots == 0 || n_slots > 0xFFFFFFFFu || threads == 0)
+        return set_error(ctx, SPX_EINVAL, "spx_atomic_probe: bad arguments");
+    return atomic_probe(ctx, buf, n_slots, ops, threads, (hipStream_t)stream);
 }
 
 void spx_ctx_destroy(spx_ctx *ctx) {
@@ -169,6 +250,7 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     hipSetDevice(ctx->device);
     hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
+    if (ctx->replica) hipFree(ctx->replica);
     if (ctx->batch_dev) hipFree(ctx->batch_dev);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
     if (ctx->batch_done) hipEventDestroy(ctx->batch_done);
@@ -322,6 +404,8 @@ static int photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *b
     p.dispersion = o.dispersion;
     p.max_depth = o.max_depth;
     p.ill_scale = illuminant_scale(ctx, o.illuminant);
+    p.replicas = 1;
+    p.rep_stride = 0;
     p.scene = scene->dev;
     p.tbl = table_for(ctx, o.illuminant);
     return SPX_OK;
@@ -357,9 +441,12 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
     if (engine == SPX_ENGINE_PLAIN) {
         e = launch_photon_plain(p, 1, s);
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
-        uint32_t chunk = 1u << 22;
+        uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        e = run_photon_wavefront(ctx, p, 1, ctx->scratch, chunk, s);
+        e = with_replicas(ctx, p, (size_t)3 * screen->width * screen->height, s,
+                          [&](const PhotonParams &q) { return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s); },
+                          &rc);
+        if (rc) return rc;
     } else {
         return set_error(ctx, SPX_EINVAL, "spx_lighttrace: unknown engine");
     }
@@ -393,9 +480,12 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
     if (engine == SPX_ENGINE_PLAIN || estimator == SPX_EST_BEAM) {
         e = launch_photon_plain(p, 0, s);
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
-        uint32_t chunk = 1u << 22;
+        uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        e = run_photon_wavefront(ctx, p, 0, ctx->scratch, chunk, s);
+        e = with_replicas(ctx, p, (size_t)3 * width * height, s,
+                          [&](const PhotonParams &q) { return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s); },
+                          &rc);
+        if (rc) return rc;
     } else {
         return set_error(ctx, SPX_EINVAL, "spx_volume: unknown engine");
     }
@@ -564,7 +654,7 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     uint32_t chunk_paths = cfg && cfg->chunk_paths ? cfg->chunk_paths : 16384u;
     uint32_t chunk_photons = cfg && cfg->chunk_photons ? cfg->chunk_photons : 8192u;
     uint32_t blocks = cfg && cfg->blocks ? cfg->blocks : (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
-    uint32_t splat_mode = cfg ? cfg->splat_mode : 0u;
+    uint32_t splat_mode = cfg ? cfg->splat_mode : ctx->tuning.splat_mode;
     if (splat_mode > 1) return set_error(ctx, SPX_EINVAL, "spx_batch_run: unknown splat mode");
     std::vector<BatchJob> table(n_jobs);
     std::vector<BatchItem> items;
@@ -600,6 +690,41 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         if (films[k].first == films[k - 1].first && (films[k].second == SPX_JOB_CAMERA || films[k - 1].second == SPX_JOB_CAMERA))
             return set_error(ctx, SPX_EINVAL, "spx_batch_run: a camera job shares its film with another job");
     if (items.empty()) return SPX_OK;
+    // Film replicas for the photon jobs (one region per job in ctx->replica).
+    std::vector<ReplicaJob> reps;
+    size_t rep_total = 0;
+    uint32_t rep_blocks = 0;
+    for (uint32_t j = 0; j < n_jobs; ++j) {
+        if (jobs[j].kind == SPX_JOB_CAMERA || jobs[j].n_photons == 0) continue;
+        size_t n = jobs[j].kind == SPX_JOB_LIGHTTRACE ? (size_t)3 * jobs[j].screen.width * jobs[j].screen.height
+                                               : (size_t)3 * jobs[j].width * jobs[j].height;
+        uint32_t R = replica_count(ctx, n);
+        if (R <= 1) continue;
+        ReplicaJob rj;
+        rj.dst = jobs[j].film;
+        rj.src = (const float *)(uintptr_t)rep_total;  // offset, rebased below
+        rj.n = (uint32_t)n;
+        rj.reps = R;
+        rj.first_block = rep_blocks;
+        rj.pad = j;
+        reps.push_back(rj);
+        rep_total += (size_t)R * n;
+        rep_blocks += (uint32_t)((n + 1023) / 1024);
+    }
+    if (!reps.empty()) {
+        int rc2 = ensure_replicas(ctx, rep_total, s);
+        if (rc2) return rc2;
+        for (ReplicaJob &rj : reps) {
+            size_t off = (size_t)(uintptr_t)rj.src;
+            rj.src = ctx->replica + off;
+            PhotonParams &p = table[rj.pad].ph;
+            p.film = ctx->replica + off;
+            p.replicas = rj.reps;
+            p.rep_stride = rj.n;
+        }
+        hipError_t ef = launch_fill(ctx->replica, 0.0f, rep_total, s);
+        if (ef != hipSuccess) return check_hip(ctx, ef, "spx_batch_run replica clear");
+    }
     // Jobs to XCD queues, largest first onto the least-loaded queue.
     std::vector<uint32_t> order(n_jobs);
     for (uint32_t j = 0; j < n_jobs; ++j) order[j] = j;
@@ -624,13 +749,15 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     for (uint32_t q = 1; q < kXcdQueues; ++q) qh.begin[q] = qh.begin[q - 1] + qh.size[q - 1];
     size_t off_items = (sizeof(BatchQueues) + 255) & ~(size_t)255;
     size_t off_jobs = (off_items + items.size() * sizeof(BatchItem) + 255) & ~(size_t)255;
-    size_t bytes = off_jobs + table.size() * sizeof(BatchJob);
+    size_t off_reps = (off_jobs + table.size() * sizeof(BatchJob) + 255) & ~(size_t)255;
+    size_t bytes = off_reps + reps.size() * sizeof(ReplicaJob);
     int rc = batch_stage(ctx, bytes, s);
     if (rc) return rc;
     char *h = (char *)ctx->batch_host;
     memcpy(h, &qh, sizeof qh);
     memcpy(h + off_items, items.data(), items.size() * sizeof(BatchItem));
     memcpy(h + off_jobs, table.data(), table.size() * sizeof(BatchJob));
+    if (!reps.empty()) memcpy(h + off_reps, reps.data(), reps.size() * sizeof(ReplicaJob));
     char *d = (char *)ctx->batch_dev;
     hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
@@ -638,6 +765,8 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     ctx->batch_pending = true;
     e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items), (BatchQueues *)d, blocks,
                      splat_mode, s);
+    if (e == hipSuccess && !reps.empty())
+        e = launch_reduce_replicas((const ReplicaJob *)(d + off_reps), (uint32_t)reps.size(), rep_blocks, s);
     return check_hip(ctx, e, "spx_batch_run launch");
 }
 

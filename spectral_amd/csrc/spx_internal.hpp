@@ -29,7 +29,9 @@ struct PhotonParams {
     float scr_u[3], scr_vv;
     float scr_v[3], pad0;
     float scr_n[3], pad1;  // lighttrace: Screen::normal
-    uint32_t scr_w, scr_h, pad2, pad3;
+    uint32_t scr_w, scr_h;
+    uint32_t replicas;     // film copies receiving deposits (>= 1); copy r = film + r * rep_stride
+    uint32_t rep_stride;   // floats between copies
     spx_vol_params vp;
     uint32_t w, h, n, photon_base;
     uint32_t seed, estimator, illuminant, dispersion;
@@ -54,6 +56,9 @@ struct spx_ctx {
     void *scratch = nullptr;   // grow-only work buffer for the wavefront queues
     size_t scratch_bytes = 0;
     hipStream_t scratch_stream = nullptr;
+    float *replica = nullptr;     // grow-only film replica buffer
+    size_t replica_floats = 0;
+    spx_tuning tuning;
     void *batch_dev = nullptr;    // grow-only device job table + items + queues
     size_t batch_dev_bytes = 0;
     void *batch_host = nullptr;   // pinned staging for the job table
@@ -93,6 +98,23 @@ hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *o
 hipError_t launch_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s);
 hipError_t launch_copy(float *dst, const float *src, size_t n, hipStream_t s);
+
+// Film replicas: photon deposits of one job are spread over R copies (block b
+// writes copy b % R) to divide same-pixel atomic contention by R; the copies
+// are then summed into the caller's film.
+struct ReplicaJob {
+    float *dst;         // caller's film
+    const float *src;   // R copies of n floats each
+    uint32_t n, reps;
+    uint32_t first_block, pad;
+};
+hipError_t launch_reduce_replicas(const ReplicaJob *jobs_dev, uint32_t n_jobs, uint32_t total_blocks,
+                                  hipStream_t s);
+int ensure_replicas(spx_ctx *ctx, size_t floats, hipStream_t stream);
+hipError_t launch_reduce_one(const ReplicaJob &J, hipStream_t s);
+// Replica count for a film of n floats under the context's tuning.
+uint32_t replica_count(const spx_ctx *ctx, size_t film_floats);
+int atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, hipStream_t s);
 
 // Wavefront engine (per-bounce SoA queues).
 size_t camera_wavefront_scratch(uint32_t chunk);

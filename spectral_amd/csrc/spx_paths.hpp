@@ -344,6 +344,11 @@ __device__ inline bool volume_bounce(const SceneData &S, const float4 *tbl, cons
     return true;
 }
 
+// The film copy this block deposits into.
+__device__ inline float *deposit_film(const PhotonParams &P) {
+    return P.replicas > 1 ? P.film + (size_t)(blockIdx.x % P.replicas) * P.rep_stride : P.film;
+}
+
 // Sinks ---------------------------------------------------------------------
 struct GlobalSink {
     float *film;

@@ -107,7 +107,7 @@ def slab_scene() -> Scene:
 # ---------------------------------------------------------------------------
 PRISM_R = float(F(1.0 / (2.0 * math.sqrt(3.0))))  # inradius of the side-1.0 triangle
 PRISM_NORMALS = ((-0.8660254, 0.5, 0.0), (0.8660254, 0.5, 0.0), (0.0, -1.0, 0.0))
-FLOOR_Y = -0.3
+FLOOR_Y = -1.6  # below the beam (which starts near y = -1.49)
 
 
 def prism_planes(rotation_deg: float = 0.0):
@@ -124,7 +124,9 @@ def prism_scene(rotation_deg: float = 0.0, floor: bool = False) -> Scene:
     sc = Scene(background=0.0)
     sc.add_solid(prism_planes(rotation_deg), Dielectric(A.GLASS_SF11))
     if floor:
-        sc.add_solid([((0.0, 1.0, 0.0), -FLOOR_Y)], Lambertian(0.8))
+        # A thick bounded slab, so that no ray ever starts inside an unbounded
+        # half-space (geom.rs:157-171 would then report a hit at t = +inf).
+        sc.add_solid(axis_box((-40.0, FLOOR_Y - 1.0, -40.0), (40.0, FLOOR_Y, 40.0)), Lambertian(0.8))
     return sc
 
 
@@ -150,15 +152,15 @@ def prism_camera(width: int = 320, height: int = 180, orbit_deg: float = 0.0):
 
 
 def floor_camera(width: int = 1920, height: int = 1080, orbit_deg: float = 0.0):
-    tgt = np.array([0.6, FLOOR_Y, 0.0])
+    tgt = np.array([0.8, -0.8, 0.0])
     a = math.radians(orbit_deg)
-    org = tgt + np.array([4.6 * math.sin(a), 1.5, 4.6 * math.cos(a)])
+    org = tgt + np.array([6.0 * math.sin(a), 2.3, 6.0 * math.cos(a)])
     return Camera.look_at(org, tgt, (0, 1, 0), 40.0, width / height)
 
 
 def floor_screen(w: int = 1024, h: int = 1024):
     """The floor as a lighttrace Screen (prism_on_plane.rs:81-87 pattern): normal +y."""
-    return screen((-2.0, FLOOR_Y, -2.5), (0.0, 0.0, 5.0), (6.0, 0.0, 0.0), w, h)
+    return screen((-2.0, FLOOR_Y, -3.0), (0.0, 0.0, 6.0), (8.0, 0.0, 0.0), w, h)
 
 
 HAZE = dict(sigma_s=0.05, sigma_t=0.05, g=0.0, max_dist=14.0)
