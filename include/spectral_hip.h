@@ -275,6 +275,56 @@ typedef struct {
 int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg,
                   void *stream);
 
+/* ---- serving: continuous batching over resident films ------------------------- */
+/* A request is K frames of one job.  Frame f traces the job with its RNG
+ * counter advanced by f frames (sample_base + f*spp, photon_base + f*n, as
+ * tracer.rs:119 continues the sample index), its camera turned by
+ * f*orbit_deg about the vertical axis through orbit_center.  With
+ * orbit_deg == 0 the frames accumulate into one progressive film; otherwise
+ * the film is reset before each frame (a camera move resets accumulation,
+ * as the viewer does).  The server owns each request's film from admission
+ * until spx_server_release. */
+typedef struct {
+    spx_job job;        /* frame-0 job; job.film is ignored */
+    uint32_t frames;    /* K >= 1 */
+    float orbit_deg;
+    float orbit_center[3];
+} spx_request;
+
+#define SPX_REQ_PENDING 0u
+#define SPX_REQ_ACTIVE 1u
+#define SPX_REQ_DONE 2u
+
+typedef struct {
+    uint32_t state;       /* SPX_REQ_* */
+    uint32_t frames_done;
+    uint64_t footprint;   /* film + deposit-replica bytes reserved at admission */
+    double t_submit;      /* seconds on the server clock */
+    double t_admit;
+    double t_done;
+} spx_request_info;
+
+typedef struct spx_server spx_server;
+
+int spx_server_create(spx_ctx *ctx, uint64_t mem_cap_bytes, spx_server **out);
+void spx_server_destroy(spx_server *srv);
+/* Queues a request (validated now; an invalid request fails alone with
+ * SPX_EINVAL).  SPX_EBUSY if its footprint exceeds the whole cap. */
+int spx_server_submit(spx_server *srv, const spx_request *req, uint64_t *id);
+/* One continuous-batching iteration: admits pending requests (oldest first)
+ * whose footprint fits under the cap, runs the next frame of every active
+ * request in ONE batched launch on `stream`, waits for it and retires the
+ * requests that finished.  *n_active (optional) = requests in that launch. */
+int spx_server_step(spx_server *srv, void *stream, uint32_t *n_active);
+int spx_server_info(const spx_server *srv, uint64_t id, spx_request_info *out);
+/* The camera basis frame `frame` of a request uses. */
+int spx_server_frame_camera(const spx_server *srv, uint64_t id, uint32_t frame, spx_camera *out);
+/* Copies a request's film (3*w*h floats) to host memory.  Synchronous. */
+int spx_server_film(spx_server *srv, uint64_t id, float *host_out, size_t floats);
+int spx_server_release(spx_server *srv, uint64_t id);
+uint64_t spx_server_bytes_in_use(const spx_server *srv);
+double spx_server_clock(const spx_server *srv);
+
 /* ---- device memory and streams (so callers need no HIP bindings) --------------- */
 int spx_malloc(spx_ctx *ctx, size_t bytes, void **out);
 int spx_free(spx_ctx *ctx, void *ptr);

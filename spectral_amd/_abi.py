@@ -96,6 +96,18 @@ class BatchCfg(C.Structure):
                 ("chunk_photons", C.c_uint32)]
 
 
+class Request(C.Structure):
+    _fields_ = [("job", Job), ("frames", C.c_uint32), ("orbit_deg", C.c_float),
+                ("orbit_center", F3)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_uint32), ("frames_done", C.c_uint32), ("footprint", C.c_uint64),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+REQ_PENDING, REQ_ACTIVE, REQ_DONE = 0, 1, 2
+
 STRUCT_SIZES = {"prim": 32, "plane": 16, "material": 16, "env": 32, "camera": 48, "beam": 48,
                 "screen": 44, "opts": 16, "vol_params": 16, "photon_record": 304}
 
@@ -128,6 +140,16 @@ SIGNATURES = {
     "spx_rng_draws": (_i32, [_vp, _u32, _u32, _u32, _P(_u32)]),
     "spx_glass_n": (_i32, [_vp, _u32, _P(_f32), _u32, _P(_f32)]),
     "spx_batch_run": (_i32, [_vp, _P(Job), _u32, _P(BatchCfg), _vp]),
+    "spx_server_create": (_i32, [_vp, C.c_uint64, _P(_vp)]),
+    "spx_server_destroy": (None, [_vp]),
+    "spx_server_submit": (_i32, [_vp, _P(Request), _P(C.c_uint64)]),
+    "spx_server_step": (_i32, [_vp, _vp, _P(_u32)]),
+    "spx_server_info": (_i32, [_vp, C.c_uint64, _P(RequestInfo)]),
+    "spx_server_frame_camera": (_i32, [_vp, C.c_uint64, _u32, _P(CameraBasis)]),
+    "spx_server_film": (_i32, [_vp, C.c_uint64, _P(_f32), _sz]),
+    "spx_server_release": (_i32, [_vp, C.c_uint64]),
+    "spx_server_bytes_in_use": (C.c_uint64, [_vp]),
+    "spx_server_clock": (C.c_double, [_vp]),
     "spx_malloc": (_i32, [_vp, _sz, _P(_vp)]),
     "spx_free": (_i32, [_vp, _vp]),
     "spx_memcpy_h2d": (_i32, [_vp, _vp, _vp, _sz, _vp]),

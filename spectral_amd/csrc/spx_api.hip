@@ -844,3 +844,10 @@ int spx_event_elapsed_ms(spx_ctx *ctx, void *a, void *b, float *ms) {
 }
 
 }  // extern "C"
+
+namespace spx {
+int validate_job(spx_ctx *ctx, const spx_job &job) {
+    BatchJob tmp;
+    return build_batch_job(ctx, job, tmp, 0);
+}
+}  // namespace spx

@@ -114,6 +114,8 @@ int ensure_replicas(spx_ctx *ctx, size_t floats, hipStream_t stream);
 hipError_t launch_reduce_one(const ReplicaJob &J, hipStream_t s);
 // Replica count for a film of n floats under the context's tuning.
 uint32_t replica_count(const spx_ctx *ctx, size_t film_floats);
+// Validates a batch job exactly as spx_batch_run would (no launch).
+int validate_job(spx_ctx *ctx, const spx_job &job);
 int atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, hipStream_t s);
 
 // Wavefront engine (per-bounce SoA queues).
