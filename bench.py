@@ -177,6 +177,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a 256 MiB device copy 4x (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the light-transport jobs are independent and this "
@@ -190,6 +193,14 @@ def main() -> int:
     ctx = S.Context(0)
     stream = ctx.stream()
     timer = Timer(ctx, stream)
+    if args.copy_calib:
+        n = 64 << 20
+        a, b = S.DeviceBuffer(ctx, n), S.DeviceBuffer(ctx, n)
+        for _ in range(4):
+            A.lib().spx_copy_f32(ctx.handle, b.ptr, a.ptr, n, stream)
+        ctx.sync(stream)
+        print(json.dumps({"copy_calib_bytes_read": 4 * n, "copy_calib_bytes_written": 4 * n}), flush=True)
+        return 0
     specs = jobs.mixed_jobs()
     paths = sum(s.units for s in specs if s.kind == A.JOB_CAMERA)
     photons = sum(s.units for s in specs if s.kind == A.JOB_VOLUME)
@@ -204,7 +215,10 @@ def main() -> int:
     ms_step, wall_ms = timer.run(lambda: br.run_batched(stream, cfg), args.steps)
     value = total / ms_step / 1e3
     log(f"batched: {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
-    ms_plain, _ = timer.run(lambda: br.run_plain(stream), 1)
+    if args.no_plain:
+        ms_plain = float("nan")
+    else:
+        ms_plain, _ = timer.run(lambda: br.run_plain(stream), 1)
     plain_value = total / ms_plain / 1e3
     log(f"plain one-job-per-launch: {ms_plain:.2f} ms -> {plain_value:.1f}")
     br.close()
@@ -221,7 +235,8 @@ def main() -> int:
         "config": {"workload": "512 mixed jobs: rainbow camera renders + prism-on-floor volume renders",
                    "jobs": len(specs), "paths": paths, "photons": photons, "engine": "batched",
                    "generator_seed": 0},
-        "plain_value": round(plain_value, 2), "batched_over_plain": round(value / plain_value, 3),
+        "plain_value": None if args.no_plain else round(plain_value, 2),
+        "batched_over_plain": None if args.no_plain else round(value / plain_value, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_batch", "work_model": {k: round(v, 4) for k, v in wm.items()}},
