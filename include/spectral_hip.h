@@ -159,8 +159,10 @@ typedef struct spx_scene spx_scene;
 typedef struct {
     uint32_t max_replicas;   /* photon film copies per job (1 = deposit straight into the film) */
     uint32_t replica_mib;    /* per-job budget for the copies, MiB */
-    uint32_t splat_mode;     /* 0 = direct fp32 atomics, 1 = LDS hash first */
+    uint32_t splat_mode;     /* photon deposits: 0 = direct fp32 atomics, 1 = LDS hash first,
+                                2 = binned records (count/scan/scatter/accumulate, no atomics) */
     uint32_t chunk_log2;     /* wavefront chunk: 2^chunk_log2 paths / photons */
+    uint32_t bin_log2;       /* binned deposits: 2^bin_log2 records per round (16 B each, x2) */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */
@@ -262,7 +264,7 @@ typedef struct {
 } spx_job;
 
 typedef struct {
-    uint32_t splat_mode;   /* photon deposits: 0 = direct fp32 atomics, 1 = LDS hash first */
+    uint32_t splat_mode;   /* photon deposits: as spx_tuning.splat_mode */
     uint32_t blocks;       /* persistent grid size (0 = tuned default) */
     uint32_t chunk_paths;  /* camera work item size in paths (0 = tuned default) */
     uint32_t chunk_photons;/* photon work item size (0 = tuned default) */

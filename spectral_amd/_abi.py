@@ -88,7 +88,7 @@ class Job(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("max_replicas", C.c_uint32), ("replica_mib", C.c_uint32),
-                ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32)]
+                ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32), ("bin_log2", C.c_uint32)]
 
 
 class BatchCfg(C.Structure):

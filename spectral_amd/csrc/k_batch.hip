@@ -85,7 +85,7 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
 
 __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs,
                                                const BatchItem *__restrict__ items,
-                                               BatchQueues *q, uint32_t splat_mode) {
+                                               BatchQueues *q, uint32_t splat_mode, BinState bins) {
     __shared__ SceneData S;
     __shared__ float4 tbl[kTblN];
     __shared__ uint32_t sk[kBatchSlots];
@@ -94,6 +94,7 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
     __shared__ uint32_t s_item;
     __shared__ const SceneData *s_scene;
     __shared__ const float4 *s_tbl;
+    __shared__ uint32_t wstate[2 * 4];  // binned deposits: {chunk, fill} per wave
     float *acc = sv;  // camera items reuse the splat value array as the pixel accumulator
     LdsSplat<kBatchSlots> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
@@ -101,6 +102,8 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
         s_tbl = nullptr;
     }
     hash.init();
+    if (threadIdx.x < 8) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
+    const uint32_t pool = xcc_id() % kBinPools;
     uint32_t home = blockIdx.x % kXcdQueues;
     uint32_t qi = 0;  // queues tried so far (home first, then the others)
     while (true) {
@@ -146,6 +149,10 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
             if (splat_mode == 0) {
                 GlobalSink g{deposit_film(P)};
                 batch_photon_item(S, tbl, P, kind, item.start, item.count, g);
+            } else if (splat_mode == 2) {
+                BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
+                          wstate, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w};
+                batch_photon_item(S, tbl, P, kind, item.start, item.count, b);
             } else {
                 hash.film = deposit_film(P);
                 batch_photon_item(S, tbl, P, kind, item.start, item.count, hash);
@@ -154,11 +161,12 @@ __global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs
         }
         __syncthreads();
     }
+    if (splat_mode == 2) bin_wave_finish(wstate, bins.chunk_fill);
 }
 
 hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueues *q, uint32_t blocks,
-                        uint32_t splat_mode, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(256), 0, s, jobs, items, q, splat_mode);
+                        uint32_t splat_mode, const BinState &bins, hipStream_t s) {
+    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(256), 0, s, jobs, items, q, splat_mode, bins);
     return hipGetLastError();
 }
 

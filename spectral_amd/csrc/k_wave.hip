@@ -10,6 +10,8 @@
 #include "spx_paths.hpp"
 #include "spx_splat.hpp"
 
+#include <string.h>
+
 namespace spx {
 
 // Live-path compaction: one atomic per wave, slot = base + prefix(ballot).
@@ -227,68 +229,79 @@ struct PhWave {
 constexpr int kSplatSlots = 2048;
 
 // MODE: 0 volume point, 2 volume legacy, 3 lighttrace.
-template <int MODE>
+template <int MODE, class Sink>
 __device__ inline bool ph_bounce(const SceneData &S, const float4 *tbl, const PhotonParams &P,
-                                 PhotonState &st, LdsSplat<kSplatSlots> &sink) {
+                                 PhotonState &st, Sink &sink) {
     if (MODE == 3) return lighttrace_bounce(S, tbl, P, st, sink) == 0;
     if (MODE == 2) return volume_bounce<SPX_EST_LEGACY>(S, tbl, P, st, sink);
     return volume_bounce<SPX_EST_POINT>(S, tbl, P, st, sink);
 }
 
-#define SPX_SPLAT_LDS                                    \
-    __shared__ uint32_t sk[kSplatSlots];                 \
-    __shared__ float sv[3 * kSplatSlots];                \
-    __shared__ uint32_t sfill;                           \
-    LdsSplat<kSplatSlots> sink{sk, sv, &sfill, deposit_film(P)};
+// Per-kernel deposit sink: SINK 1 = LDS hash + atomics, 2 = binned records.
+template <int SINK>
+struct PhSink;
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ph_first(PhotonParams P, PhWave W, uint32_t start, uint32_t count) {
-    __shared__ SceneData S;
-    __shared__ float4 tbl[kTblN];
-    SPX_SPLAT_LDS
-    stage_scene(S, P.scene, tbl, P.tbl);
-    sink.init();
-    __syncthreads();
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // Block-uniform trip count so that the flush barriers line up.
-    for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
-        uint32_t j = base + threadIdx.x;
-        if (j < count) {
-            PhotonState st;
-            photon_begin(P, tbl, P.photon_base + start + j, st);
-            bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && P.max_depth > 1;
-            uint32_t at = wave_enqueue(alive, W.counts + 1);
-            if (alive) ph_store(W.q[1], at, st);
-        }
-        sink.maybe_flush();
-    }
-    sink.flush();
-}
+template <>
+struct PhSink<1> {
+    uint32_t sk[kSplatSlots];
+    float sv[3 * kSplatSlots];
+    uint32_t sfill;
+};
 
-template <int MODE>
-__global__ __launch_bounds__(256) void k_ph_bounce(PhotonParams P, PhWave W, uint32_t b) {
-    __shared__ SceneData S;
-    __shared__ float4 tbl[kTblN];
-    SPX_SPLAT_LDS
-    stage_scene(S, P.scene, tbl, P.tbl);
-    sink.init();
-    __syncthreads();
-    const uint32_t n_in = W.counts[b];
+template <>
+struct PhSink<2> {
+    uint32_t wstate[2 * 4];
+};
+
+template <int MODE, int SINK>
+__device__ inline void ph_body(const SceneData &S, const float4 *tbl, const PhotonParams &P, PhWave &W,
+                               uint32_t b, uint32_t start, uint32_t count, const BinState &bins,
+                               PhSink<SINK> &L) {
+    const bool first = b == 0;
+    const uint32_t n_in = first ? count : W.counts[b];
     const PhQueue &qin = W.q[b & 1u];
     const PhQueue &qout = W.q[(b + 1u) & 1u];
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
-        uint32_t i = base + threadIdx.x;
-        if (i < n_in) {
-            PhotonState st;
-            ph_load(qin, i, st);
-            bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
-            uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
-            if (alive) ph_store(qout, at, st);
+    auto step = [&](auto &sink) {
+        // Block-uniform trip count so that the flush barriers line up.
+        for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
+            uint32_t i = base + threadIdx.x;
+            if (i < n_in) {
+                PhotonState st;
+                if (first) photon_begin(P, tbl, P.photon_base + start + i, st);
+                else ph_load(qin, i, st);
+                bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
+                uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
+                if (alive) ph_store(qout, at, st);
+            }
+            if constexpr (SINK == 1) sink.maybe_flush();
         }
-        sink.maybe_flush();
+    };
+    if constexpr (SINK == 1) {
+        LdsSplat<kSplatSlots> sink{L.sk, L.sv, &L.sfill, deposit_film(P)};
+        sink.init();
+        __syncthreads();
+        step(sink);
+        sink.flush();
+    } else {
+        if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
+        __syncthreads();
+        BinSink sink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, xcc_id() % kBinPools, P.key_base,
+                     P.film, L.wstate, MODE == 3 ? P.scr_w : P.w};
+        step(sink);
+        bin_wave_finish(L.wstate, bins.chunk_fill);
     }
-    sink.flush();
+}
+
+template <int MODE, int SINK>
+__global__ __launch_bounds__(256) void k_ph(PhotonParams P, PhWave W, uint32_t b, uint32_t start, uint32_t count,
+                                            BinState bins) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    __shared__ PhSink<SINK> L;
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    ph_body<MODE, SINK>(S, tbl, P, W, b, start, count, bins, L);
 }
 
 size_t photon_wavefront_scratch(uint32_t chunk) { return 2 * 9 * align_up((size_t)chunk * 4) + 256; }
@@ -314,24 +327,36 @@ static PhWave carve_ph(void *scratch, uint32_t chunk) {
     return W;
 }
 
-template <int MODE>
-static void run_ph(spx_ctx *ctx, const PhotonParams &p, PhWave W, uint32_t chunk, hipStream_t s) {
+template <int MODE, int SINK>
+static void run_ph(spx_ctx *ctx, const PhotonParams &p, PhWave W, uint32_t chunk, hipStream_t s,
+                   const BinState *bins, const uint32_t *job_shared) {
+    BinState none;
+    memset(&none, 0, sizeof none);
+    const BinState &B = bins ? *bins : none;
     for (uint32_t start = 0; start < p.n; start += chunk) {
         uint32_t count = p.n - start < chunk ? p.n - start : chunk;
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
+        if (SINK == 2) launch_bins_reset(B, s);
         uint32_t g = persistent_blocks(ctx, count, 4);
-        hipLaunchKernelGGL(k_ph_first<MODE>, dim3(g), dim3(256), 0, s, p, W, start, count);
-        for (uint32_t b = 1; b < p.max_depth; ++b)
-            hipLaunchKernelGGL(k_ph_bounce<MODE>, dim3(g), dim3(256), 0, s, p, W, b);
+        for (uint32_t b = 0; b < p.max_depth; ++b)
+            hipLaunchKernelGGL((k_ph<MODE, SINK>), dim3(g), dim3(256), 0, s, p, W, b, start, count, B);
+        if (SINK == 2) launch_bins_resolve(B, job_shared, (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4, s);
     }
 }
 
 hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
-                                uint32_t chunk, hipStream_t s) {
+                                uint32_t chunk, hipStream_t s, const BinState *bins,
+                                const uint32_t *job_shared) {
     PhWave W = carve_ph(scratch, chunk);
-    if (lighttrace) run_ph<3>(ctx, p, W, chunk, s);
-    else if (p.estimator == SPX_EST_LEGACY) run_ph<2>(ctx, p, W, chunk, s);
-    else run_ph<0>(ctx, p, W, chunk, s);
+    if (bins) {
+        if (lighttrace) run_ph<3, 2>(ctx, p, W, chunk, s, bins, job_shared);
+        else if (p.estimator == SPX_EST_LEGACY) run_ph<2, 2>(ctx, p, W, chunk, s, bins, job_shared);
+        else run_ph<0, 2>(ctx, p, W, chunk, s, bins, job_shared);
+    } else {
+        if (lighttrace) run_ph<3, 1>(ctx, p, W, chunk, s, bins, job_shared);
+        else if (p.estimator == SPX_EST_LEGACY) run_ph<2, 1>(ctx, p, W, chunk, s, bins, job_shared);
+        else run_ph<0, 1>(ctx, p, W, chunk, s, bins, job_shared);
+    }
     return hipGetLastError();
 }
 

@@ -8,6 +8,7 @@
 #include <string>
 
 #include <algorithm>
+#include <map>
 #include <vector>
 
 #include "spx_batch.hpp"
@@ -86,6 +87,104 @@ uint32_t replica_count(const spx_ctx *ctx, size_t film_floats) {
     size_t r = budget / (film_floats * sizeof(float));
     if (r > t.max_replicas) r = t.max_replicas;
     return r < 2 ? 1u : (uint32_t)r;
+}
+
+int upload_async(spx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return SPX_OK;
+    if (ctx->stage_pending) {
+        hipError_t e = hipEventSynchronize(ctx->stage_done);
+        if (e != hipSuccess) return check_hip(ctx, e, "upload wait");
+        ctx->stage_pending = false;
+    }
+    if (!ctx->stage_done) {
+        hipError_t e = hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming);
+        if (e != hipSuccess) return check_hip(ctx, e, "upload event");
+    }
+    if (ctx->stage_bytes < bytes) {
+        if (ctx->stage) hipHostFree(ctx->stage);
+        ctx->stage = nullptr;
+        ctx->stage_bytes = 0;
+        hipError_t e = hipHostMalloc(&ctx->stage, bytes * 2, hipHostMallocDefault);
+        if (e != hipSuccess) return check_hip(ctx, e, "upload staging");
+        ctx->stage_bytes = bytes * 2;
+    }
+    memcpy(ctx->stage, host, bytes);
+    hipError_t e = hipMemcpyAsync(dev, ctx->stage, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipEventRecord(ctx->stage_done, s);
+    if (e != hipSuccess) return check_hip(ctx, e, "upload copy");
+    ctx->stage_pending = true;
+    return SPX_OK;
+}
+
+static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st, const uint32_t **job_shared) {
+    const size_t cap = (size_t)1 << ctx->tuning.bin_log2;
+    const size_t chunks = cap / kBinChunk;
+    const size_t n = map.n_strips, nj = map.film.size();
+    size_t off = 0;
+    auto take = [&](size_t bytes) { size_t o = off; off += a256(bytes); return o; };
+    size_t o_recs = take(cap * sizeof(BinRec)), o_sorted = take(cap * sizeof(BinRec));
+    size_t o_pool = take(kBinPools * 4), o_partn = take(4), o_fill = take(chunks * 4);
+    size_t o_cnt = take(n * 4), o_off = take((n + 1) * 4), o_cur = take(n * 4), o_poff = take((n + 1) * 4);
+    // the uploaded map: strip_job | job_key | job_npix | job_shared | job_film
+    size_t o_map = off;
+    size_t m_sj = 0, m_key = a256(n * 4), m_npix = m_key + a256(nj * 4), m_w = m_npix + a256(nj * 4),
+           m_sh = m_w + a256(nj * 4), m_film = m_sh + a256(nj * 4), m_end = m_film + a256(nj * sizeof(float *));
+    off += m_end;
+    if (ctx->bins_bytes < off) {
+        hipError_t e = hipDeviceSynchronize();  // earlier rounds may still use the old buffers
+        if (e != hipSuccess) return check_hip(ctx, e, "prepare_bins sync");
+        if (ctx->bins) hipFree(ctx->bins);
+        ctx->bins = nullptr;
+        ctx->bins_bytes = 0;
+        size_t want = off + off / 8;
+        e = hipMalloc(&ctx->bins, want);
+        if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(bins)");
+        ctx->bins_bytes = want;
+    }
+    char *b = (char *)ctx->bins;
+    std::vector<char> host(m_end, 0);
+    memcpy(host.data() + m_sj, map.strip_job.data(), n * 4);
+    memcpy(host.data() + m_key, map.key.data(), nj * 4);
+    memcpy(host.data() + m_npix, map.npix.data(), nj * 4);
+    memcpy(host.data() + m_w, map.width.data(), nj * 4);
+    memcpy(host.data() + m_sh, map.shared.data(), nj * 4);
+    memcpy(host.data() + m_film, map.film.data(), nj * sizeof(float *));
+    int rc = upload_async(ctx, b + o_map, host.data(), m_end, s);
+    if (rc) return rc;
+    st.recs = (BinRec *)(b + o_recs);
+    st.sorted = (BinRec *)(b + o_sorted);
+    st.pool_next = (uint32_t *)(b + o_pool);
+    st.chunk_fill = (uint32_t *)(b + o_fill);
+    st.strip_count = (uint32_t *)(b + o_cnt);
+    st.strip_off = (uint32_t *)(b + o_off);
+    st.strip_cur = (uint32_t *)(b + o_cur);
+    st.part_off = (uint32_t *)(b + o_poff);
+    st.part_next = (uint32_t *)(b + o_partn);
+    st.strip_job = (const uint32_t *)(b + o_map + m_sj);
+    st.job_key = (const uint32_t *)(b + o_map + m_key);
+    st.job_npix = (const uint32_t *)(b + o_map + m_npix);
+    st.job_width = (const uint32_t *)(b + o_map + m_w);
+    st.job_film = (float *const *)(b + o_map + m_film);
+    st.pool_cap = (uint32_t)(cap / kBinPools);
+    st.n_strips = (uint32_t)n;
+    *job_shared = (const uint32_t *)(b + o_map + m_sh);
+    return SPX_OK;
+}
+
+// Single-job photon call through binned deposits.
+template <class F>
+static hipError_t with_bins(spx_ctx *ctx, PhotonParams &p, uint32_t w, uint32_t h, hipStream_t s, F launch,
+                            int *rc) {
+    BinJobMap map;
+    p.key_base = map.add(p.film, w, h);
+    p.replicas = 1;
+    p.rep_stride = 0;
+    BinState st;
+    const uint32_t *shared = nullptr;
+    if ((*rc = prepare_bins(ctx, map, s, st, &shared))) return hipSuccess;
+    return launch(p, &st, shared);
 }
 
 // Runs `launch` with deposits spread over film replicas, then sums them
@@ -220,14 +319,15 @@ int spx_ctx_create(int device, spx_ctx **out) {
     ctx->tuning.replica_mib = 64;
     ctx->tuning.splat_mode = 1;
     ctx->tuning.chunk_log2 = 22;
+    ctx->tuning.bin_log2 = 29;
     *out = ctx;
     return SPX_OK;
 }
 
 int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
     if (!ctx || !t) return SPX_EINVAL;
-    if (t->max_replicas < 1 || t->max_replicas > 256 || t->splat_mode > 1 || t->chunk_log2 < 12 ||
-        t->chunk_log2 > 26 || t->replica_mib > 65536)
+    if (t->max_replicas < 1 || t->max_replicas > 256 || t->splat_mode > 2 || t->chunk_log2 < 12 ||
+        t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30)
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
     return SPX_OK;
@@ -251,6 +351,9 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->replica) hipFree(ctx->replica);
+    if (ctx->bins) hipFree(ctx->bins);
+    if (ctx->stage) hipHostFree(ctx->stage);
+    if (ctx->stage_done) hipEventDestroy(ctx->stage_done);
     if (ctx->batch_dev) hipFree(ctx->batch_dev);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
     if (ctx->batch_done) hipEventDestroy(ctx->batch_done);
@@ -443,9 +546,16 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        e = with_replicas(ctx, p, (size_t)3 * screen->width * screen->height, s,
-                          [&](const PhotonParams &q) { return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s); },
-                          &rc);
+        if (ctx->tuning.splat_mode == 2)
+            e = with_bins(ctx, p, screen->width, screen->height, s,
+                          [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
+                              return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s, b, sh);
+                          }, &rc);
+        else
+            e = with_replicas(ctx, p, (size_t)3 * screen->width * screen->height, s,
+                              [&](const PhotonParams &q) {
+                                  return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s, nullptr, nullptr);
+                              }, &rc);
         if (rc) return rc;
     } else {
         return set_error(ctx, SPX_EINVAL, "spx_lighttrace: unknown engine");
@@ -482,9 +592,16 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        e = with_replicas(ctx, p, (size_t)3 * width * height, s,
-                          [&](const PhotonParams &q) { return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s); },
-                          &rc);
+        if (ctx->tuning.splat_mode == 2)
+            e = with_bins(ctx, p, width, height, s,
+                          [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
+                              return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s, b, sh);
+                          }, &rc);
+        else
+            e = with_replicas(ctx, p, (size_t)3 * width * height, s,
+                              [&](const PhotonParams &q) {
+                                  return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s, nullptr, nullptr);
+                              }, &rc);
         if (rc) return rc;
     } else {
         return set_error(ctx, SPX_EINVAL, "spx_volume: unknown engine");
@@ -655,7 +772,8 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     uint32_t chunk_photons = cfg && cfg->chunk_photons ? cfg->chunk_photons : 8192u;
     uint32_t blocks = cfg && cfg->blocks ? cfg->blocks : (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
     uint32_t splat_mode = cfg ? cfg->splat_mode : ctx->tuning.splat_mode;
-    if (splat_mode > 1) return set_error(ctx, SPX_EINVAL, "spx_batch_run: unknown splat mode");
+    if (splat_mode > 2) return set_error(ctx, SPX_EINVAL, "spx_batch_run: unknown splat mode");
+    const bool binned = splat_mode == 2;
     std::vector<BatchJob> table(n_jobs);
     std::vector<BatchItem> items;
     std::vector<double> job_cost(n_jobs, 0.0);
@@ -685,19 +803,65 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         }
     }
     // Camera films are written without atomics: no two jobs may share one.
-    std::sort(films.begin(), films.end());
-    for (size_t k = 1; k < films.size(); ++k)
-        if (films[k].first == films[k - 1].first && (films[k].second == SPX_JOB_CAMERA || films[k - 1].second == SPX_JOB_CAMERA))
+    std::vector<std::pair<const float *, uint32_t>> sorted_films = films;
+    std::sort(sorted_films.begin(), sorted_films.end());
+    for (size_t k = 1; k < sorted_films.size(); ++k)
+        if (sorted_films[k].first == sorted_films[k - 1].first &&
+            (sorted_films[k].second == SPX_JOB_CAMERA || sorted_films[k - 1].second == SPX_JOB_CAMERA))
             return set_error(ctx, SPX_EINVAL, "spx_batch_run: a camera job shares its film with another job");
     if (items.empty()) return SPX_OK;
-    // Film replicas for the photon jobs (one region per job in ctx->replica).
+
+    // Binned deposits: key ranges per photon job; photon items split into
+    // rounds of at most ~85 % of the record capacity (estimated deposits:
+    // 14 per volume photon, 4 per lighttrace photon; overflow falls back to
+    // atomics, so the estimate only affects speed).
+    BinJobMap map;
+    std::vector<uint32_t> item_round(items.size(), 0);
+    uint32_t n_rounds = 1;
+    if (binned) {
+        std::map<const float *, uint32_t> first_owner;
+        for (uint32_t j = 0; j < n_jobs; ++j) {
+            if (jobs[j].kind == SPX_JOB_CAMERA) continue;
+            const bool lt = jobs[j].kind == SPX_JOB_LIGHTTRACE;
+            uint32_t fw = lt ? jobs[j].screen.width : jobs[j].width, fh = lt ? jobs[j].screen.height : jobs[j].height;
+            if (((uint64_t)map.n_strips + film_strips(fw, fh)) * kStripPix > 0xFFFFFFFFull)
+                return set_error(ctx, SPX_ECAP, "spx_batch_run: photon films exceed 2^32 tile pixels");
+            table[j].ph.key_base = map.add(jobs[j].film, fw, fh);
+            table[j].ph.replicas = 1;
+            table[j].ph.rep_stride = 0;
+            auto it = first_owner.find(jobs[j].film);
+            if (it != first_owner.end()) {
+                map.shared.back() = 1;
+                map.shared[it->second] = 1;
+            } else {
+                first_owner[jobs[j].film] = (uint32_t)map.film.size() - 1;
+            }
+        }
+        const double budget = 0.85 * (double)((size_t)1 << ctx->tuning.bin_log2);
+        double used = 0.0;
+        uint32_t r = 0;
+        for (size_t k = 0; k < items.size(); ++k) {
+            const spx_job &jb = jobs[items[k].job];
+            if (jb.kind == SPX_JOB_CAMERA) continue;  // round 0
+            double est = (double)items[k].count * (jb.kind == SPX_JOB_LIGHTTRACE ? 4.0 : 14.0);
+            if (used > 0.0 && used + est > budget) {
+                ++r;
+                used = 0.0;
+            }
+            used += est;
+            item_round[k] = r;
+        }
+        n_rounds = r + 1;
+    }
+
+    // Film replicas for the photon jobs (atomic deposit modes only).
     std::vector<ReplicaJob> reps;
     size_t rep_total = 0;
     uint32_t rep_blocks = 0;
-    for (uint32_t j = 0; j < n_jobs; ++j) {
+    for (uint32_t j = 0; j < n_jobs && !binned; ++j) {
         if (jobs[j].kind == SPX_JOB_CAMERA || jobs[j].n_photons == 0) continue;
         size_t n = jobs[j].kind == SPX_JOB_LIGHTTRACE ? (size_t)3 * jobs[j].screen.width * jobs[j].screen.height
-                                               : (size_t)3 * jobs[j].width * jobs[j].height;
+                                                      : (size_t)3 * jobs[j].width * jobs[j].height;
         uint32_t R = replica_count(ctx, n);
         if (R <= 1) continue;
         ReplicaJob rj;
@@ -725,6 +889,7 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         hipError_t ef = launch_fill(ctx->replica, 0.0f, rep_total, s);
         if (ef != hipSuccess) return check_hip(ctx, ef, "spx_batch_run replica clear");
     }
+
     // Jobs to XCD queues, largest first onto the least-loaded queue.
     std::vector<uint32_t> order(n_jobs);
     for (uint32_t j = 0; j < n_jobs; ++j) order[j] = j;
@@ -738,24 +903,45 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         job_queue[j] = best;
         load[best] += job_cost[j];
     }
-    std::stable_sort(items.begin(), items.end(), [&](const BatchItem &a, const BatchItem &b) {
-        uint32_t qa = job_queue[a.job], qb = job_queue[b.job];
+    std::vector<uint32_t> idx(items.size());
+    for (size_t k = 0; k < items.size(); ++k) idx[k] = (uint32_t)k;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
+        if (item_round[a] != item_round[b]) return item_round[a] < item_round[b];
+        uint32_t qa = job_queue[items[a].job], qb = job_queue[items[b].job];
         if (qa != qb) return qa < qb;
-        return a.cost > b.cost;
+        return items[a].cost > items[b].cost;
     });
-    BatchQueues qh;
-    memset(&qh, 0, sizeof qh);
-    for (const BatchItem &it : items) qh.size[job_queue[it.job]]++;
-    for (uint32_t q = 1; q < kXcdQueues; ++q) qh.begin[q] = qh.begin[q - 1] + qh.size[q - 1];
-    size_t off_items = (sizeof(BatchQueues) + 255) & ~(size_t)255;
-    size_t off_jobs = (off_items + items.size() * sizeof(BatchItem) + 255) & ~(size_t)255;
+    std::vector<BatchItem> ordered(items.size());
+    std::vector<BatchQueues> qh(n_rounds);
+    memset(qh.data(), 0, n_rounds * sizeof(BatchQueues));
+    for (size_t k = 0; k < idx.size(); ++k) {
+        ordered[k] = items[idx[k]];
+        qh[item_round[idx[k]]].size[job_queue[items[idx[k]].job]]++;
+    }
+    uint32_t start = 0;
+    for (uint32_t r = 0; r < n_rounds; ++r)
+        for (uint32_t q = 0; q < kXcdQueues; ++q) {
+            qh[r].begin[q] = start;
+            start += qh[r].size[q];
+        }
+    const size_t qstride = (sizeof(BatchQueues) + 255) & ~(size_t)255;
+    size_t off_items = qstride * n_rounds;
+    size_t off_jobs = (off_items + ordered.size() * sizeof(BatchItem) + 255) & ~(size_t)255;
     size_t off_reps = (off_jobs + table.size() * sizeof(BatchJob) + 255) & ~(size_t)255;
     size_t bytes = off_reps + reps.size() * sizeof(ReplicaJob);
+    BinState bst;
+    memset(&bst, 0, sizeof bst);
+    const uint32_t *job_shared = nullptr;
+    if (binned) {
+        int rc2 = prepare_bins(ctx, map, s, bst, &job_shared);
+        if (rc2) return rc2;
+    }
     int rc = batch_stage(ctx, bytes, s);
     if (rc) return rc;
     char *h = (char *)ctx->batch_host;
-    memcpy(h, &qh, sizeof qh);
-    memcpy(h + off_items, items.data(), items.size() * sizeof(BatchItem));
+    memset(h, 0, bytes);
+    for (uint32_t r = 0; r < n_rounds; ++r) memcpy(h + r * qstride, &qh[r], sizeof(BatchQueues));
+    memcpy(h + off_items, ordered.data(), ordered.size() * sizeof(BatchItem));
     memcpy(h + off_jobs, table.data(), table.size() * sizeof(BatchJob));
     if (!reps.empty()) memcpy(h + off_reps, reps.data(), reps.size() * sizeof(ReplicaJob));
     char *d = (char *)ctx->batch_dev;
@@ -763,8 +949,14 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
     if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run upload");
     ctx->batch_pending = true;
-    e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items), (BatchQueues *)d, blocks,
-                     splat_mode, s);
+    const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
+    for (uint32_t r = 0; r < n_rounds && e == hipSuccess; ++r) {
+        if (binned) e = launch_bins_reset(bst, s);
+        if (e == hipSuccess)
+            e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items),
+                             (BatchQueues *)(d + r * qstride), blocks, splat_mode, bst, s);
+        if (e == hipSuccess && binned) e = launch_bins_resolve(bst, job_shared, acc_blocks, s);
+    }
     if (e == hipSuccess && !reps.empty())
         e = launch_reduce_replicas((const ReplicaJob *)(d + off_reps), (uint32_t)reps.size(), rep_blocks, s);
     return check_hip(ctx, e, "spx_batch_run launch");

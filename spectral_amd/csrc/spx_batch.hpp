@@ -29,6 +29,6 @@ struct BatchQueues {
 };
 
 hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueues *q, uint32_t blocks,
-                        uint32_t splat_mode, hipStream_t s);
+                        uint32_t splat_mode, const BinState &bins, hipStream_t s);
 
 }  // namespace spx

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "spectral_hip.h"
+#include "spx_bins.hpp"
 #include "spx_device.hpp"
 
 namespace spx {
@@ -35,7 +36,8 @@ struct PhotonParams {
     spx_vol_params vp;
     uint32_t w, h, n, photon_base;
     uint32_t seed, estimator, illuminant, dispersion;
-    uint32_t max_depth, pad4;
+    uint32_t max_depth;
+    uint32_t key_base;     // binned deposits: first record key of this job's film
     float ill_scale;       // photon power weight S(lambda)/mean S (1 for equal energy)
     float pad5;
     const SceneData *scene;
@@ -59,6 +61,12 @@ struct spx_ctx {
     float *replica = nullptr;     // grow-only film replica buffer
     size_t replica_floats = 0;
     spx_tuning tuning;
+    void *bins = nullptr;         // grow-only binned-deposit buffers
+    size_t bins_bytes = 0;
+    void *stage = nullptr;        // pinned staging for small stream-ordered uploads
+    size_t stage_bytes = 0;
+    hipEvent_t stage_done = nullptr;
+    bool stage_pending = false;
     void *batch_dev = nullptr;    // grow-only device job table + items + queues
     size_t batch_dev_bytes = 0;
     void *batch_host = nullptr;   // pinned staging for the job table
@@ -114,6 +122,39 @@ int ensure_replicas(spx_ctx *ctx, size_t floats, hipStream_t stream);
 hipError_t launch_reduce_one(const ReplicaJob &J, hipStream_t s);
 // Replica count for a film of n floats under the context's tuning.
 uint32_t replica_count(const spx_ctx *ctx, size_t film_floats);
+// Binned deposits: per-call job map (key ranges -> films) and the passes.
+struct BinJobMap {
+    std::vector<uint32_t> key, npix, width, shared, strip_job;
+    std::vector<float *> film;
+    uint32_t n_strips = 0;
+    // Adds a film of w x h pixels; returns its key base.
+    uint32_t add(float *f, uint32_t w, uint32_t h) {
+        uint32_t k = n_strips << kStripLog2;
+        key.push_back(k);
+        npix.push_back(w * h);
+        width.push_back(w);
+        shared.push_back(0);
+        film.push_back(f);
+        uint32_t t = 1u << kTileLog2;
+        uint32_t strips = ((w + t - 1) >> kTileLog2) * ((h + t - 1) >> kTileLog2);
+        for (uint32_t i = 0; i < strips; ++i) strip_job.push_back((uint32_t)film.size() - 1);
+        n_strips += strips;
+        return k;
+    }
+};
+// Allocates the round buffers for `map` (capacity from the tuning), uploads
+// the map on `s` and fills `st` / `job_shared`.
+int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st, const uint32_t **job_shared);
+hipError_t launch_bins_reset(const BinState &st, hipStream_t s);
+// Tile grid size of a film, in strips (32x32 tiles).
+inline uint32_t film_strips(uint32_t w, uint32_t h) {
+    uint32_t t = 1u << kTileLog2;
+    return ((w + t - 1) >> kTileLog2) * ((h + t - 1) >> kTileLog2);
+}
+hipError_t launch_bins_resolve(const BinState &st, const uint32_t *job_shared, uint32_t acc_blocks, hipStream_t s);
+// Stream-ordered upload of a small host block through pinned staging.
+int upload_async(spx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStream_t s);
+
 // Validates a batch job exactly as spx_batch_run would (no launch).
 int validate_job(spx_ctx *ctx, const spx_job &job);
 int atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, hipStream_t s);
@@ -124,6 +165,7 @@ hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch,
                                 hipStream_t s);
 size_t photon_wavefront_scratch(uint32_t chunk);
 hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
-                                uint32_t chunk, hipStream_t s);
+                                uint32_t chunk, hipStream_t s, const BinState *bins,
+                                const uint32_t *job_shared);
 
 }  // namespace spx

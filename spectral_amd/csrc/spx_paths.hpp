@@ -231,8 +231,8 @@ __device__ inline void volume_point_segment(const PhotonParams &P, const float4 
             float d = fmaxf(length(to), 1e-3f);
             float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
             if (d < zb) {
-                float cos_theta = dot(st.rd, to / d);
-                float phase = phase_hg(P.vp.g, cos_theta);
+                // g == 0: phase_hg(0, c) = 1 / (4 pi * 1^1.5) for every c, bit for bit.
+                float phase = P.vp.g == 0.0f ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
                 float trans = expf(-P.vp.sigma_t * d);
                 float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
                 sink.add(idx, cmf * contrib);

@@ -19,10 +19,14 @@ def _small_mix(n=16):
     return jobs.mixed_jobs(n=n, seed=3, max_side=160, max_photons_log2=16, max_spp=12)
 
 
-@pytest.mark.parametrize("splat_mode", [0, 1])
-def test_batch_equals_alone(ctx, splat_mode):
+@pytest.mark.parametrize("splat_mode,bin_log2", [(0, 27), (1, 27), (2, 27), (2, 15)])
+def test_batch_equals_alone(ctx, splat_mode, bin_log2):
+    # bin_log2 = 15 (32768 records per round) forces many rounds and pool
+    # overflow into the atomic fallback.
     specs = _small_mix()
     br = jobs.BatchRunner(ctx, specs)
+    old = ctx.tuning()
+    ctx.set_tuning(bin_log2=bin_log2)
     try:
         br.clear()
         br.run_batched(cfg=A.BatchCfg(splat_mode, 0, 4096, 2048))
@@ -35,15 +39,17 @@ def test_batch_equals_alone(ctx, splat_mode):
             err = rel_l1_per_channel(a, b)
             assert (err < ALONE_TOL).all(), f"job {j} ({specs[j]}): {err}"
     finally:
+        ctx.set_tuning(bin_log2=old.bin_log2)
         br.close()
 
 
-def test_batch_matches_oracle(ctx, oracle):
+@pytest.mark.parametrize("splat_mode", [1, 2])
+def test_batch_matches_oracle(ctx, oracle, splat_mode):
     specs = _small_mix(8)
     br = jobs.BatchRunner(ctx, specs)
     try:
         br.clear()
-        br.run_batched()
+        br.run_batched(cfg=A.BatchCfg(splat_mode, 0, 0, 0))
         for j, sp in enumerate(specs):
             g = br.films[j].image()
             sc, cam, bm = jobs.job_geometry(sp)

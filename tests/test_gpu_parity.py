@@ -279,3 +279,33 @@ def test_invalid_arguments_fail_loudly(ctx):
     tr = S.HipTracer(ctx, scenes.diff_gate_scene(), scenes.gate_camera(), 8, 8)
     tr.accumulate(2)
     assert tr.sum().sum() > 0
+
+
+@pytest.mark.parametrize("bin_log2", [27, 15])
+def test_binned_deposits_wavefront(ctx, oracle, bin_log2):
+    # splat mode 2: deposits as records, sorted by strip and accumulated in
+    # LDS (no film atomics); bin_log2 = 15 overflows the pools into the
+    # atomic fallback.
+    old = ctx.tuning()
+    ctx.set_tuning(splat_mode=2, bin_log2=bin_log2)
+    try:
+        sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera()
+        vp = A.VolParams(**scenes.HAZE)
+        g, c = _volume_pair(ctx, oracle, sc, cam, bm, 320, 180, 1 << 18, vp, 0, ill=A.ILLUM_D65,
+                            engine=A.ENGINE_WAVEFRONT)
+        err = rel_l1_per_channel(g, c)
+        print("binned prism volume", bin_log2, err)
+        assert (err < FILM_TOL).all()
+        w, h = 192, 108
+        fsc, fcam = scenes.prism_scene(floor=True), scenes.floor_camera(w, h)
+        _, depth = oracle.surface_rim(fsc, fcam, w, h)
+        g, c = _volume_pair(ctx, oracle, fsc, fcam, bm, w, h, 1 << 16, vp, 3, ill=A.ILLUM_A,
+                            engine=A.ENGINE_WAVEFRONT, zbuf=depth.ravel())
+        assert (rel_l1_per_channel(g, c) < FILM_TOL).all()
+        scr = scenes.floor_screen(256, 256)
+        film = S.lighttrace_trace(ctx, sc, scr, bm, 1 << 17, 0x0F1, illuminant=A.ILLUM_D65,
+                                  engine=A.ENGINE_WAVEFRONT)
+        c, _ = oracle.lighttrace(sc, scr, bm, 1 << 17, 0, 0x0F1, opts(A.ILLUM_D65), threads=8)
+        assert (rel_l1_per_channel(film.image(), c) < FILM_TOL).all()
+    finally:
+        ctx.set_tuning(splat_mode=old.splat_mode, bin_log2=old.bin_log2)
