@@ -79,6 +79,21 @@ def work_model():
     return {"seg_per_path": seg_per_path, "seg_per_photon": segs / n, "splat_per_photon": splats / n}
 
 
+def profiled_traffic():
+    """HBM bytes per k_batch launch on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "k_batch_mixed512_hbm_counters.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(paths_total, photons_total, target_s=12.0):
     """The oracle on this host: camera paths on all cores (the reference's
     rayon loop), volume photons single-threaded (volume.rs:118)."""
@@ -212,7 +227,11 @@ def main() -> int:
     for _ in range(args.warmup):
         br.run_batched(stream, cfg)
     ctx.sync(stream)
+    ctx.kernel_timing(True)
     ms_step, wall_ms = timer.run(lambda: br.run_batched(stream, cfg), args.steps)
+    k_ms, k_launches = ctx.kernel_time()
+    ctx.kernel_timing(False)
+    k_ms_step = k_ms / args.steps  # k_batch time per step (one launch per deposit round)
     value = total / ms_step / 1e3
     log(f"batched: {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
     if args.no_plain:
@@ -226,7 +245,8 @@ def main() -> int:
     wm = work_model()
     alg_bytes = paths * (wm["seg_per_path"] * CAM_SEG_BYTES + CAM_PATH_BYTES) + photons * (
         wm["seg_per_photon"] * PH_SEG_BYTES + wm["splat_per_photon"] * SPLAT_BYTES)
-    achieved = alg_bytes / (ms_step / 1e3) / 1e9
+    achieved = alg_bytes / (k_ms_step / 1e3) / 1e9
+    traffic = profiled_traffic()
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": UNIT, "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
@@ -238,8 +258,10 @@ def main() -> int:
         "plain_value": None if args.no_plain else round(plain_value, 2),
         "batched_over_plain": None if args.no_plain else round(value / plain_value, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_batch", "work_model": {k: round(v, 4) for k, v in wm.items()}},
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "kernel": "k_batch", "kernel_ms_per_launch": round(k_ms / max(k_launches, 1), 3),
+                     "launches_per_step": k_launches // args.steps, "alg_bytes": round(alg_bytes),
+                     "work_model": {k: round(v, 4) for k, v in wm.items()}},
     }
     if not args.no_configs:
         result["configs"] = per_config(ctx, timer, args.quick)

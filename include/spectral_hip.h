@@ -160,9 +160,11 @@ typedef struct {
     uint32_t max_replicas;   /* photon film copies per job (1 = deposit straight into the film) */
     uint32_t replica_mib;    /* per-job budget for the copies, MiB */
     uint32_t splat_mode;     /* photon deposits: 0 = direct fp32 atomics, 1 = LDS hash first,
-                                2 = binned records (count/scan/scatter/accumulate, no atomics) */
+                                2 = binned records sorted by 32x32 tile and summed in LDS as
+                                fixed point (no fp32 atomics; films bit-reproducible); default 2 */
     uint32_t chunk_log2;     /* wavefront chunk: 2^chunk_log2 paths / photons */
-    uint32_t bin_log2;       /* binned deposits: 2^bin_log2 records per round (16 B each, x2) */
+    uint32_t bin_log2;       /* binned deposits: at most 2^bin_log2 records per round (32 B of
+                                device memory each; buffers sized to the work); default 30 */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */
@@ -175,6 +177,13 @@ const char *spx_last_error(const spx_ctx *ctx);
 int spx_ctx_num_cus(const spx_ctx *ctx);
 int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t);
 int spx_ctx_get_tuning(const spx_ctx *ctx, spx_tuning *t);
+/* Measurement hook (no counterpart in the reference): while enabled, every
+ * launch of the batched engine's trace kernel (k_batch) is bracketed by HIP
+ * events on the caller's stream.  spx_ctx_kernel_time waits for the launches
+ * recorded so far, returns their summed duration (ms) and count, and clears
+ * the record.  Enabling clears it too. */
+int spx_ctx_kernel_timing(spx_ctx *ctx, int enable);
+int spx_ctx_kernel_time(spx_ctx *ctx, double *total_ms, uint32_t *launches);
 
 /* Camera::look_at (camera.rs:15-27), evaluated on the host.  Needs no device. */
 int spx_camera_look_at(const float origin[3], const float target[3], const float up[3],

@@ -122,6 +122,8 @@ SIGNATURES = {
     "spx_ctx_num_cus": (_i32, [_vp]),
     "spx_ctx_set_tuning": (_i32, [_vp, _P(Tuning)]),
     "spx_ctx_get_tuning": (_i32, [_vp, _P(Tuning)]),
+    "spx_ctx_kernel_timing": (_i32, [_vp, _i32]),
+    "spx_ctx_kernel_time": (_i32, [_vp, _P(C.c_double), _P(_u32)]),
     "spx_atomic_probe": (_i32, [_vp, _vp, _sz, _u32, _u32, _vp]),
     "spx_camera_look_at": (_i32, [_P(_f32), _P(_f32), _P(_f32), _f32, _f32, _P(CameraBasis)]),
     "spx_scene_create": (_i32, [_vp, _P(Env), _P(Prim), _u32, _P(Plane), _u32, _P(Material), _u32,

@@ -52,6 +52,16 @@ class Context:
             setattr(t, k, int(v))
         _check(self, self._lib.spx_ctx_set_tuning(self.handle, C.byref(t)))
 
+    def kernel_timing(self, enable: bool) -> None:
+        """Bracket each batched trace-kernel launch with HIP events."""
+        _check(self, self._lib.spx_ctx_kernel_timing(self.handle, int(bool(enable))))
+
+    def kernel_time(self):
+        """(summed ms, launches) of the trace-kernel launches since the last read."""
+        ms, n = C.c_double(0.0), C.c_uint32(0)
+        _check(self, self._lib.spx_ctx_kernel_time(self.handle, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def close(self) -> None:
         if self.handle:
             self._lib.spx_ctx_destroy(self.handle)

@@ -118,15 +118,26 @@ int upload_async(spx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStr
 
 static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st, const uint32_t **job_shared) {
-    const size_t cap = (size_t)1 << ctx->tuning.bin_log2;
+int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
+                 const uint32_t **job_shared) {
+    // Capacity: the smallest power of two >= want, within [2^15, 2^bin_log2].
+    size_t cap = (size_t)1 << 15;
+    while (cap < want && cap < ((size_t)1 << ctx->tuning.bin_log2)) cap <<= 1;
     const size_t chunks = cap / kBinChunk;
     const size_t n = map.n_strips, nj = map.film.size();
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += a256(bytes); return o; };
     size_t o_recs = take(cap * sizeof(BinRec)), o_sorted = take(cap * sizeof(BinRec));
+    const size_t groups = cap / kGroupRecs;
+    const size_t zero_words = n * (2 * kSub + 3);  // sub_count | sub_max | ovf_count | ovf_max | ovf_cur
     size_t o_pool = take(kBinPools * 4), o_partn = take(4), o_fill = take(chunks * 4);
-    size_t o_cnt = take(n * 4), o_off = take((n + 1) * 4), o_cur = take(n * 4), o_poff = take((n + 1) * 4);
+    size_t o_zero = take(zero_words * 4), o_tot = take(n * 4), o_max = take(n * 4);
+    size_t o_off = take((n + 1) * 4), o_poff = take((n + 1) * 4);
+    size_t o_glist = take(groups * kGroupSlots * sizeof(uint2)), o_gcount = take(groups * 4);
+    // A split strip holds more than kPartRecs records, so at most cap / kPartRecs exist.
+    const size_t max_multi = cap / kPartRecs + 1;
+    size_t o_macc = take(max_multi * 3 * kStripPix * 8), o_smacc = take(n * 4), o_maccs = take(max_multi * 4),
+           o_maccn = take(4);
     // the uploaded map: strip_job | job_key | job_npix | job_shared | job_film
     size_t o_map = off;
     size_t m_sj = 0, m_key = a256(n * 4), m_npix = m_key + a256(nj * 4), m_w = m_npix + a256(nj * 4),
@@ -157,10 +168,24 @@ int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st
     st.sorted = (BinRec *)(b + o_sorted);
     st.pool_next = (uint32_t *)(b + o_pool);
     st.chunk_fill = (uint32_t *)(b + o_fill);
-    st.strip_count = (uint32_t *)(b + o_cnt);
+    uint32_t *z = (uint32_t *)(b + o_zero);
+    st.sub_count = z;
+    st.sub_max = z + n * kSub;
+    st.ovf_count = z + 2 * n * kSub;
+    st.ovf_max = st.ovf_count + n;
+    st.ovf_cur = st.ovf_max + n;
+    st.zero_words = zero_words;
+    st.strip_total = (uint32_t *)(b + o_tot);
+    st.strip_max = (uint32_t *)(b + o_max);
     st.strip_off = (uint32_t *)(b + o_off);
-    st.strip_cur = (uint32_t *)(b + o_cur);
     st.part_off = (uint32_t *)(b + o_poff);
+    st.glist = (uint2 *)(b + o_glist);
+    st.gcount = (uint32_t *)(b + o_gcount);
+    st.macc = (unsigned long long *)(b + o_macc);
+    st.strip_macc = (uint32_t *)(b + o_smacc);
+    st.macc_strip = (uint32_t *)(b + o_maccs);
+    st.macc_count = (uint32_t *)(b + o_maccn);
+    st.max_multi = (uint32_t)max_multi;
     st.part_next = (uint32_t *)(b + o_partn);
     st.strip_job = (const uint32_t *)(b + o_map + m_sj);
     st.job_key = (const uint32_t *)(b + o_map + m_key);
@@ -173,17 +198,19 @@ int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st
     return SPX_OK;
 }
 
-// Single-job photon call through binned deposits.
+// Single-job photon call through binned deposits; `photons` is the most
+// one deposit round holds (the wavefront resolves once per chunk).
 template <class F>
-static hipError_t with_bins(spx_ctx *ctx, PhotonParams &p, uint32_t w, uint32_t h, hipStream_t s, F launch,
-                            int *rc) {
+static hipError_t with_bins(spx_ctx *ctx, PhotonParams &p, uint32_t w, uint32_t h, uint32_t photons, int lighttrace,
+                            hipStream_t s, F launch, int *rc) {
     BinJobMap map;
     p.key_base = map.add(p.film, w, h);
     p.replicas = 1;
     p.rep_stride = 0;
     BinState st;
     const uint32_t *shared = nullptr;
-    if ((*rc = prepare_bins(ctx, map, s, st, &shared))) return hipSuccess;
+    const size_t want = (size_t)((double)photons * records_per_photon(lighttrace) / kBinFill);
+    if ((*rc = prepare_bins(ctx, map, want, s, st, &shared))) return hipSuccess;
     return launch(p, &st, shared);
 }
 
@@ -317,9 +344,9 @@ int spx_ctx_create(int device, spx_ctx **out) {
     }
     ctx->tuning.max_replicas = 16;
     ctx->tuning.replica_mib = 64;
-    ctx->tuning.splat_mode = 1;
+    ctx->tuning.splat_mode = 2;
     ctx->tuning.chunk_log2 = 22;
-    ctx->tuning.bin_log2 = 29;
+    ctx->tuning.bin_log2 = 30;
     *out = ctx;
     return SPX_OK;
 }
@@ -357,8 +384,35 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     if (ctx->batch_dev) hipFree(ctx->batch_dev);
     if (ctx->batch_host) hipHostFree(ctx->batch_host);
     if (ctx->batch_done) hipEventDestroy(ctx->batch_done);
+    for (hipEvent_t ev : ctx->ktime_ev) hipEventDestroy(ev);
     if (ctx->tbl_dev) hipFree(ctx->tbl_dev);
     delete ctx;
+}
+
+int spx_ctx_kernel_timing(spx_ctx *ctx, int enable) {
+    if (!ctx) return SPX_EINVAL;
+    ctx->ktime_on = enable != 0;
+    ctx->ktime_used = 0;
+    return SPX_OK;
+}
+
+int spx_ctx_kernel_time(spx_ctx *ctx, double *total_ms, uint32_t *launches) {
+    if (!ctx || !total_ms || !launches) return set_error(ctx, SPX_EINVAL, "spx_ctx_kernel_time: null argument");
+    double sum = 0.0;
+    for (size_t k = 0; k + 1 < ctx->ktime_used; k += 2) {
+        hipError_t e = hipEventSynchronize(ctx->ktime_ev[k + 1]);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ktime_ev[k], ctx->ktime_ev[k + 1]);
+        if (e != hipSuccess) {
+            ctx->ktime_used = 0;
+            return check_hip(ctx, e, "spx_ctx_kernel_time");
+        }
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = (uint32_t)(ctx->ktime_used / 2);
+    ctx->ktime_used = 0;
+    return SPX_OK;
 }
 
 const char *spx_last_error(const spx_ctx *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
@@ -547,7 +601,7 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
         if (ctx->tuning.splat_mode == 2)
-            e = with_bins(ctx, p, screen->width, screen->height, s,
+            e = with_bins(ctx, p, screen->width, screen->height, std::min(n, chunk), 1, s,
                           [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
                               return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s, b, sh);
                           }, &rc);
@@ -593,7 +647,7 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
         if (ctx->tuning.splat_mode == 2)
-            e = with_bins(ctx, p, width, height, s,
+            e = with_bins(ctx, p, width, height, std::min(n, chunk), 0, s,
                           [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
                               return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s, b, sh);
                           }, &rc);
@@ -812,10 +866,11 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     if (items.empty()) return SPX_OK;
 
     // Binned deposits: key ranges per photon job; photon items split into
-    // rounds of at most ~85 % of the record capacity (estimated deposits:
-    // 14 per volume photon, 4 per lighttrace photon; overflow falls back to
-    // atomics, so the estimate only affects speed).
+    // rounds of at most kBinFill of the record capacity (estimated deposits
+    // per photon from records_per_photon; overflow falls back to atomics, so
+    // the estimate only affects speed).
     BinJobMap map;
+    size_t bin_want = 0;
     std::vector<uint32_t> item_round(items.size(), 0);
     uint32_t n_rounds = 1;
     if (binned) {
@@ -837,21 +892,23 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
                 first_owner[jobs[j].film] = (uint32_t)map.film.size() - 1;
             }
         }
-        const double budget = 0.85 * (double)((size_t)1 << ctx->tuning.bin_log2);
-        double used = 0.0;
+        const double budget = kBinFill * (double)((size_t)1 << ctx->tuning.bin_log2);
+        double used = 0.0, most = 0.0;
         uint32_t r = 0;
         for (size_t k = 0; k < items.size(); ++k) {
             const spx_job &jb = jobs[items[k].job];
             if (jb.kind == SPX_JOB_CAMERA) continue;  // round 0
-            double est = (double)items[k].count * (jb.kind == SPX_JOB_LIGHTTRACE ? 4.0 : 14.0);
+            double est = (double)items[k].count * records_per_photon(jb.kind == SPX_JOB_LIGHTTRACE);
             if (used > 0.0 && used + est > budget) {
                 ++r;
                 used = 0.0;
             }
             used += est;
+            most = std::max(most, used);
             item_round[k] = r;
         }
         n_rounds = r + 1;
+        bin_want = (size_t)(most / kBinFill);
     }
 
     // Film replicas for the photon jobs (atomic deposit modes only).
@@ -933,7 +990,7 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     memset(&bst, 0, sizeof bst);
     const uint32_t *job_shared = nullptr;
     if (binned) {
-        int rc2 = prepare_bins(ctx, map, s, bst, &job_shared);
+        int rc2 = prepare_bins(ctx, map, bin_want, s, bst, &job_shared);
         if (rc2) return rc2;
     }
     int rc = batch_stage(ctx, bytes, s);
@@ -952,9 +1009,24 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
     for (uint32_t r = 0; r < n_rounds && e == hipSuccess; ++r) {
         if (binned) e = launch_bins_reset(bst, s);
+        hipEvent_t ev_end = nullptr;
+        if (e == hipSuccess && ctx->ktime_on) {
+            while (ctx->ktime_ev.size() < ctx->ktime_used + 2) {
+                hipEvent_t ev;
+                e = hipEventCreate(&ev);
+                if (e != hipSuccess) break;
+                ctx->ktime_ev.push_back(ev);
+            }
+            if (e == hipSuccess) e = hipEventRecord(ctx->ktime_ev[ctx->ktime_used], s);
+            if (e == hipSuccess) ev_end = ctx->ktime_ev[ctx->ktime_used + 1];
+        }
         if (e == hipSuccess)
             e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items),
                              (BatchQueues *)(d + r * qstride), blocks, splat_mode, bst, s);
+        if (e == hipSuccess && ev_end) {
+            e = hipEventRecord(ev_end, s);
+            if (e == hipSuccess) ctx->ktime_used += 2;
+        }
         if (e == hipSuccess && binned) e = launch_bins_resolve(bst, job_shared, acc_blocks, s);
     }
     if (e == hipSuccess && !reps.empty())

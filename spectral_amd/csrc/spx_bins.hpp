@@ -1,16 +1,22 @@
 // spx_bins.hpp -- binned photon deposits: records instead of film atomics.
 //
 // Deposits are appended as 16-byte records {key, X, Y, Z} (key = the job's
-// key base + pixel) into chunked pools, one pool per XCD.  Each wave owns
+// key base + tile-ordered pixel) into chunked pools, one pool per XCD (a
+// wave whose pool is full takes chunks from the others).  Each wave owns
 // one 512-record chunk at a time, so an append costs one LDS read-modify-
 // write by the wave's leader and one coalesced 16-byte store per lane; a new
 // chunk costs one global atomic per 512 records.  After the trace, the
 // records are counted per strip (a 32x32-pixel film tile: spatially
 // compact, so the few tiles a beam or caustic crosses collect its deposits),
-// scanned, scattered into strip order and accumulated strip by strip in LDS; a strip's film pixels are
-// then written with plain stores (one owner) or, for strips split over
-// several blocks, with one fp32 atomic per pixel and block.  A full pool
-// falls back to direct film atomics, so results never depend on capacity.
+// scanned, scattered into strip order and accumulated strip by strip in LDS
+// as 64-bit fixed point scaled to the strip's largest deposit and record
+// count (integer LDS atomics: order-independent, resolution finer than the
+// fp32 rounding of a single add).  A strip's film pixels are written with
+// plain stores by their one owner; strips split over several blocks first
+// sum the blocks' fixed-point tiles with 64-bit integer atomics.  Films are
+// therefore bit-reproducible unless two jobs share one (fp32 atomics between
+// jobs) or a pool fills: a full pool falls back to direct film atomics, so
+// results never depend on capacity.
 #pragma once
 
 #include "spx_device.hpp"
@@ -24,6 +30,10 @@ constexpr uint32_t kTileLog2 = 5;        // tile side 32
 constexpr uint32_t kStripPix = 1u << kStripLog2;
 constexpr uint32_t kBinPools = 8;        // one per XCD
 constexpr uint32_t kPartRecs = 16384;    // records per accumulation work item
+constexpr uint32_t kSub = 16;            // sub-counters per strip
+constexpr uint32_t kGroupSlots = 1024;   // distinct strips per group tracked in LDS
+constexpr uint32_t kGroupRecs = kChunksPerBlock * kBinChunk;  // records per group
+constexpr uint32_t kNoMacc = 0xFFFFFFFFu;
 constexpr uint32_t kChunkEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kChunkFull = 0xFFFFFFFEu;
 
@@ -33,16 +43,38 @@ struct __align__(16) BinRec {
 };
 
 // Device-side state of one deposit round (all pointers device memory).
+// Records are handled in groups of kChunksPerBlock chunks.  The count pass
+// tallies each group's strips in LDS and reserves the group's share of every
+// strip in one of kSub sub-counters (g % kSub: the same strip's counters are
+// spread so concurrent groups rarely meet on one address), leaving a list of
+// {strip, offset in sub-bucket} per group for the scatter pass.
 struct BinState {
     BinRec *recs;          // kBinPools * pool_cap records
     BinRec *sorted;        // same size
     uint32_t *pool_next;   // [kBinPools] records taken per pool
     uint32_t *chunk_fill;  // [kBinPools * pool_cap / kBinChunk]
-    uint32_t *strip_count; // [n_strips]
+    // Zeroed each round, contiguous: sub_count | sub_max | ovf_count | ovf_max | ovf_cur.
+    uint32_t *sub_count;   // [n_strips * kSub] counts; after the totals pass, offsets within the strip
+    uint32_t *sub_max;     // [n_strips * kSub] largest |x|,|y|,|z| bit pattern
+    uint32_t *ovf_count;   // [n_strips] records of groups whose LDS table was full; then their offset
+    uint32_t *ovf_max;     // [n_strips]
+    uint32_t *ovf_cur;     // [n_strips] scatter cursor of those records
+    size_t zero_words;     // words from sub_count to the end of ovf_cur
+    uint32_t *strip_total; // [n_strips]
+    uint32_t *strip_max;   // [n_strips]
     uint32_t *strip_off;   // [n_strips + 1]
-    uint32_t *strip_cur;   // [n_strips]
     uint32_t *part_off;    // [n_strips + 1]
     uint32_t *part_next;   // [1]
+    uint2 *glist;          // [groups * kGroupSlots] {strip, offset} per group
+    uint32_t *gcount;      // [groups] list length
+    // Strips split over several accumulation parts sum their parts' fixed-
+    // point tiles here (64-bit integer atomics: exact, order-independent),
+    // then one pass converts and writes the film.
+    unsigned long long *macc;  // [max_multi * 3 * kStripPix]
+    uint32_t *strip_macc;      // [n_strips] accumulator index, or kNoMacc
+    uint32_t *macc_strip;      // [max_multi] strip of each accumulator
+    uint32_t *macc_count;      // [1]
+    uint32_t max_multi;
     const uint32_t *strip_job;   // [n_strips] owning job segment
     const uint32_t *job_key;     // [n_jobs] first key of each job
     const uint32_t *job_npix;    // [n_jobs] pixels of each job's film
@@ -66,6 +98,23 @@ __host__ __device__ inline uint32_t tile_key(uint32_t idx, uint32_t width) {
     return (tile << kStripLog2) | local;
 }
 
+// A chunk of `pool` (this XCD's), else of another pool with room, else
+// kChunkFull.  A pool that reads full is skipped, which bounds how far
+// pool_next overshoots the capacity.  Kept out of line: the sink is inlined
+// at every deposit site of the trace kernels.
+__device__ __noinline__ uint32_t grab_chunk(uint32_t *pool_next, uint32_t pool_cap, uint32_t pool) {
+    uint32_t got = atomicAdd(pool_next + pool, kBinChunk);
+    if (got + kBinChunk <= pool_cap) return pool * pool_cap + got;
+#pragma unroll 1
+    for (uint32_t t = 1; t < kBinPools; ++t) {
+        const uint32_t pl = (pool + t) % kBinPools;
+        if (__hip_atomic_load(pool_next + pl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= pool_cap) continue;
+        got = atomicAdd(pool_next + pl, kBinChunk);
+        if (got + kBinChunk <= pool_cap) return pl * pool_cap + got;
+    }
+    return kChunkFull;
+}
+
 // A film sink that appends records.  `wstate` is LDS: {chunk, fill} per wave
 // of the block, initialised to kChunkEmpty by the kernel.
 struct BinSink {
@@ -73,7 +122,7 @@ struct BinSink {
     uint32_t *pool_next;
     uint32_t *chunk_fill;
     uint32_t pool_cap;
-    uint32_t pool;       // this block's pool (XCD id mod kBinPools)
+    uint32_t pool;       // this block's first-choice pool (XCD id mod kBinPools)
     uint32_t key_base;
     float *film;         // fallback when the pool is exhausted
     uint32_t *wstate;
@@ -91,8 +140,7 @@ struct BinSink {
             uint32_t c = wstate[2 * w], f = wstate[2 * w + 1];
             if (c != kChunkFull && (c == kChunkEmpty || f + cnt > kBinChunk)) {
                 if (c != kChunkEmpty) chunk_fill[c / kBinChunk] = f;
-                uint32_t got = atomicAdd(pool_next + pool, kBinChunk);
-                c = (got + kBinChunk <= pool_cap) ? pool * pool_cap + got : kChunkFull;
+                c = grab_chunk(pool_next, pool_cap, pool);
                 f = 0;
             }
             if (c == kChunkFull) {

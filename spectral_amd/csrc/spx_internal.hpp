@@ -73,6 +73,9 @@ struct spx_ctx {
     size_t batch_host_bytes = 0;
     hipEvent_t batch_done = nullptr;  // recorded after the staging copy
     bool batch_pending = false;
+    bool ktime_on = false;             // bracket k_batch launches with events
+    std::vector<hipEvent_t> ktime_ev;  // pool; pairs (start, end)
+    size_t ktime_used = 0;             // events recorded since the last read
 };
 
 struct spx_scene {
@@ -142,9 +145,16 @@ struct BinJobMap {
         return k;
     }
 };
-// Allocates the round buffers for `map` (capacity from the tuning), uploads
-// the map on `s` and fills `st` / `job_shared`.
-int prepare_bins(spx_ctx *ctx, const BinJobMap &map, hipStream_t s, BinState &st, const uint32_t **job_shared);
+// Allocates the round buffers for `map` with room for `want` records (a
+// power of two within [2^15, 2^bin_log2]), uploads the map on `s` and fills
+// `st` / `job_shared`.
+int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
+                 const uint32_t **job_shared);
+// Share of the record capacity a round is planned to fill.
+constexpr double kBinFill = 0.85;
+// Expected film deposits per photon: the prism-and-floor volume renders
+// average 13.9 (point estimator), screen photons under 4.
+inline double records_per_photon(bool lighttrace) { return lighttrace ? 4.0 : 14.0; }
 hipError_t launch_bins_reset(const BinState &st, hipStream_t s);
 // Tile grid size of a film, in strips (32x32 tiles).
 inline uint32_t film_strips(uint32_t w, uint32_t h) {

@@ -281,20 +281,21 @@ def test_invalid_arguments_fail_loudly(ctx):
     assert tr.sum().sum() > 0
 
 
-@pytest.mark.parametrize("bin_log2", [27, 15])
-def test_binned_deposits_wavefront(ctx, oracle, bin_log2):
-    # splat mode 2: deposits as records, sorted by strip and accumulated in
-    # LDS (no film atomics); bin_log2 = 15 overflows the pools into the
-    # atomic fallback.
+@pytest.mark.parametrize("splat_mode,bin_log2", [(1, 30), (2, 30), (2, 15)])
+def test_deposit_modes_wavefront(ctx, oracle, splat_mode, bin_log2):
+    # splat mode 1: LDS hash in front of film atomics; mode 2: deposits as
+    # records, sorted by 32x32 tile and accumulated in LDS as fixed point (no
+    # film atomics); bin_log2 = 15 overflows the pools into the atomic
+    # fallback.
     old = ctx.tuning()
-    ctx.set_tuning(splat_mode=2, bin_log2=bin_log2)
+    ctx.set_tuning(splat_mode=splat_mode, bin_log2=bin_log2)
     try:
         sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera()
         vp = A.VolParams(**scenes.HAZE)
         g, c = _volume_pair(ctx, oracle, sc, cam, bm, 320, 180, 1 << 18, vp, 0, ill=A.ILLUM_D65,
                             engine=A.ENGINE_WAVEFRONT)
         err = rel_l1_per_channel(g, c)
-        print("binned prism volume", bin_log2, err)
+        print("prism volume", splat_mode, bin_log2, err)
         assert (err < FILM_TOL).all()
         w, h = 192, 108
         fsc, fcam = scenes.prism_scene(floor=True), scenes.floor_camera(w, h)
@@ -307,5 +308,28 @@ def test_binned_deposits_wavefront(ctx, oracle, bin_log2):
                                   engine=A.ENGINE_WAVEFRONT)
         c, _ = oracle.lighttrace(sc, scr, bm, 1 << 17, 0, 0x0F1, opts(A.ILLUM_D65), threads=8)
         assert (rel_l1_per_channel(film.image(), c) < FILM_TOL).all()
+    finally:
+        ctx.set_tuning(splat_mode=old.splat_mode, bin_log2=old.bin_log2)
+
+
+def test_binned_fixed_point_scaling_and_determinism(ctx):
+    # Binned deposits sum in 64-bit fixed point scaled to each tile's largest
+    # deposit.  sigma_s multiplies every deposit (volume.rs point estimator),
+    # so sigma_s * 2^k must give the film * 2^k exactly, for k far outside
+    # any fixed scale; and two runs must agree bit for bit.
+    old = ctx.tuning()
+    ctx.set_tuning(splat_mode=2, bin_log2=30)
+    try:
+        sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera(160, 90)
+        base = None
+        for k in (0, 0, 90, -90):
+            f = S.render_volume(ctx, sc, cam, bm, 160, 90, 1 << 16, 0.05 * 2.0 ** k, 0.05, 0.0, 14.0,
+                                None, 21, illuminant=A.ILLUM_D65, engine=A.ENGINE_WAVEFRONT)
+            img = f.image().astype(np.float64)
+            if base is None:
+                base = img
+                assert base.sum() > 0
+            else:
+                assert np.array_equal(img, base * 2.0 ** k), k
     finally:
         ctx.set_tuning(splat_mode=old.splat_mode, bin_log2=old.bin_log2)
