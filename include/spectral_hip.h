@@ -178,7 +178,7 @@ int spx_ctx_num_cus(const spx_ctx *ctx);
 int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t);
 int spx_ctx_get_tuning(const spx_ctx *ctx, spx_tuning *t);
 /* Measurement hook (no counterpart in the reference): while enabled, every
- * launch of the batched engine's trace kernel (k_batch) is bracketed by HIP
+ * launch of the batched engine's trace kernels (k_batch_camera, k_batch_photons) is bracketed by HIP
  * events on the caller's stream.  spx_ctx_kernel_time waits for the launches
  * recorded so far, returns their summed duration (ms) and count, and clears
  * the record.  Enabling clears it too. */

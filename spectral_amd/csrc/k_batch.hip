@@ -10,10 +10,13 @@
 //  * Inside an item every lane runs its own path / photon bounce by bounce
 //    and regenerates a new one the moment its current one ends, so the wave
 //    stays full even though path lengths differ (no per-bounce barrier).
+//  * Camera items and photon items run in separate kernels (each sized for
+//    its own path's registers): the camera round first, then the photon
+//    rounds.
 //  * Camera samples are summed per pixel in an LDS accumulator and written
 //    once per item (one owner per pixel: no global atomics).  Photon
-//    deposits go through the LDS splat hash and are drained with fp32 global
-//    atomics at the end of the item.
+//    deposits go to the sink the deposit mode names: binned records, the
+//    LDS splat hash drained with fp32 atomics, or direct fp32 atomics.
 #include "spx_batch.hpp"
 #include "spx_paths.hpp"
 #include "spx_splat.hpp"
@@ -83,90 +86,127 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
     }
 }
 
-__global__ __launch_bounds__(256) void k_batch(const BatchJob *__restrict__ jobs,
-                                               const BatchItem *__restrict__ items,
-                                               BatchQueues *q, uint32_t splat_mode, BinState bins) {
-    __shared__ SceneData S;
-    __shared__ float4 tbl[kTblN];
-    __shared__ uint32_t sk[kBatchSlots];
-    __shared__ float sv[3 * kBatchSlots];
+// Shared by both kernels: block-level item claim (home XCD queue first, then
+// the others) and scene staging when the item's job changes scene.
+struct BatchBlock {
+    SceneData S;
+    float4 tbl[kTblN];
+    uint32_t item;
+    const SceneData *scene;
+    const float4 *tbl_src;
+};
+
+__device__ inline uint32_t claim_item(BatchQueues *q, BatchBlock &B, uint32_t home, uint32_t &qi) {
+    if (threadIdx.x == 0) {
+        uint32_t it = 0xFFFFFFFFu;
+        while (qi < kXcdQueues) {
+            uint32_t qq = (home + qi) % kXcdQueues;
+            uint32_t k = atomicAdd(&q->next[qq], 1u);
+            if (k < q->size[qq]) {
+                it = q->begin[qq] + k;
+                break;
+            }
+            ++qi;
+        }
+        B.item = it;
+    }
+    __syncthreads();
+    return uniform_u32(B.item);
+}
+
+__device__ inline void stage_for(BatchBlock &B, const SceneData *scene, const float4 *jt) {
+    if (scene != B.scene || jt != B.tbl_src) {  // restage only when the job's scene changes
+        __syncthreads();
+        stage_scene(B.S, scene, B.tbl, jt);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            B.scene = scene;
+            B.tbl_src = jt;
+        }
+    }
+    __syncthreads();
+}
+
+// Camera items only (sized for the camera path's registers: capped at 128
+// VGPRs for four waves per SIMD, which measured 10 % faster than three).
+__global__ __launch_bounds__(256, 4) void k_batch_camera(const BatchJob *__restrict__ jobs,
+                                                      const BatchItem *__restrict__ items, BatchQueues *q) {
+    __shared__ BatchBlock B;
+    __shared__ float acc[3 * kMaxCamPix];
+    if (threadIdx.x == 0) {
+        B.scene = nullptr;
+        B.tbl_src = nullptr;
+    }
+    const uint32_t home = blockIdx.x % kXcdQueues;
+    uint32_t qi = 0;
+    while (true) {
+        const uint32_t it = claim_item(q, B, home, qi);
+        if (it == 0xFFFFFFFFu) break;
+        const BatchItem item = items[it];
+        const uint32_t j = uniform_u32(item.job);
+        const CamParams P = jobs[j].cam;
+        stage_for(B, P.scene, P.tbl);
+        batch_camera_item(B.S, B.tbl, P, item.start, item.count, acc);
+        __syncthreads();
+    }
+}
+
+// Photon items only (volume and lighttrace), deposits through sink SPLAT:
+// 0 = direct film atomics, 1 = LDS hash, 2 = binned records.
+template <int SPLAT>
+__global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restrict__ jobs,
+                                                       const BatchItem *__restrict__ items, BatchQueues *q,
+                                                       BinState bins) {
+    __shared__ BatchBlock B;
+    __shared__ uint32_t sk[SPLAT == 1 ? kBatchSlots : 1];
+    __shared__ float sv[SPLAT == 1 ? 3 * kBatchSlots : 1];
     __shared__ uint32_t sfill;
-    __shared__ uint32_t s_item;
-    __shared__ const SceneData *s_scene;
-    __shared__ const float4 *s_tbl;
     __shared__ uint32_t wstate[2 * 4];  // binned deposits: {chunk, fill} per wave
-    float *acc = sv;  // camera items reuse the splat value array as the pixel accumulator
     LdsSplat<kBatchSlots> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
-        s_scene = nullptr;
-        s_tbl = nullptr;
+        B.scene = nullptr;
+        B.tbl_src = nullptr;
     }
-    hash.init();
+    if (SPLAT == 1) hash.init();
     if (threadIdx.x < 8) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
     const uint32_t pool = xcc_id() % kBinPools;
-    uint32_t home = blockIdx.x % kXcdQueues;
-    uint32_t qi = 0;  // queues tried so far (home first, then the others)
+    const uint32_t home = blockIdx.x % kXcdQueues;
+    uint32_t qi = 0;
     while (true) {
-        if (threadIdx.x == 0) {
-            uint32_t it = 0xFFFFFFFFu;
-            while (qi < kXcdQueues) {
-                uint32_t qq = (home + qi) % kXcdQueues;
-                uint32_t k = atomicAdd(&q->next[qq], 1u);
-                if (k < q->size[qq]) {
-                    it = q->begin[qq] + k;
-                    break;
-                }
-                ++qi;
-            }
-            s_item = it;
-        }
-        __syncthreads();
-        const uint32_t it = uniform_u32(s_item);
+        const uint32_t it = claim_item(q, B, home, qi);
         if (it == 0xFFFFFFFFu) break;
         const BatchItem item = items[it];
         const uint32_t j = uniform_u32(item.job);
         const uint32_t kind = uniform_u32(jobs[j].kind);
-        const SceneData *scene = kind == SPX_JOB_CAMERA ? jobs[j].cam.scene : jobs[j].ph.scene;
-        const float4 *jt = kind == SPX_JOB_CAMERA ? jobs[j].cam.tbl : jobs[j].ph.tbl;
-        if (scene != s_scene || jt != s_tbl) {  // restage only when the job's scene changes
-            __syncthreads();
-            stage_scene(S, scene, tbl, jt);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                s_scene = scene;
-                s_tbl = jt;
-            }
-        }
-        __syncthreads();
-        if (kind == SPX_JOB_CAMERA) {
-            const CamParams P = jobs[j].cam;
-            batch_camera_item(S, tbl, P, item.start, item.count, acc);
-            __syncthreads();
-            // restore the hash's empty value array for the next photon item
-            for (uint32_t k = threadIdx.x; k < 3 * item.count; k += blockDim.x) sv[k] = 0.0f;
+        const PhotonParams P = jobs[j].ph;
+        stage_for(B, P.scene, P.tbl);
+        if (SPLAT == 0) {
+            GlobalSink g{deposit_film(P)};
+            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
+        } else if (SPLAT == 2) {
+            BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
+                      wstate, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w};
+            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
         } else {
-            const PhotonParams P = jobs[j].ph;
-            if (splat_mode == 0) {
-                GlobalSink g{deposit_film(P)};
-                batch_photon_item(S, tbl, P, kind, item.start, item.count, g);
-            } else if (splat_mode == 2) {
-                BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
-                          wstate, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w};
-                batch_photon_item(S, tbl, P, kind, item.start, item.count, b);
-            } else {
-                hash.film = deposit_film(P);
-                batch_photon_item(S, tbl, P, kind, item.start, item.count, hash);
-                hash.flush();
-            }
+            hash.film = deposit_film(P);
+            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hash);
+            hash.flush();
         }
         __syncthreads();
     }
-    if (splat_mode == 2) bin_wave_finish(wstate, bins.chunk_fill);
+    if (SPLAT == 2) bin_wave_finish(wstate, bins.chunk_fill);
 }
 
-hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueues *q, uint32_t blocks,
-                        uint32_t splat_mode, const BinState &bins, hipStream_t s) {
-    hipLaunchKernelGGL(k_batch, dim3(blocks), dim3(256), 0, s, jobs, items, q, splat_mode, bins);
+hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *items, BatchQueues *q,
+                        uint32_t blocks, uint32_t splat_mode, const BinState &bins, hipStream_t s) {
+    if (camera)
+        hipLaunchKernelGGL(k_batch_camera, dim3(blocks), dim3(256), 0, s, jobs, items, q);
+    else if (splat_mode == 0)
+        hipLaunchKernelGGL(k_batch_photons<0>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+    else if (splat_mode == 1)
+        hipLaunchKernelGGL(k_batch_photons<1>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+    else
+        hipLaunchKernelGGL(k_batch_photons<2>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
     return hipGetLastError();
 }
 

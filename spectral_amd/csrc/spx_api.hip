@@ -869,10 +869,14 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     // rounds of at most kBinFill of the record capacity (estimated deposits
     // per photon from records_per_photon; overflow falls back to atomics, so
     // the estimate only affects speed).
+    // Round 0 holds the camera items (camera kernel); photon items follow in
+    // rounds 1.. (photon kernel).
     BinJobMap map;
     size_t bin_want = 0;
-    std::vector<uint32_t> item_round(items.size(), 0);
-    uint32_t n_rounds = 1;
+    std::vector<uint32_t> item_round(items.size(), 1);
+    for (size_t k = 0; k < items.size(); ++k)
+        if (jobs[items[k].job].kind == SPX_JOB_CAMERA) item_round[k] = 0;
+    uint32_t n_rounds = 2;
     if (binned) {
         std::map<const float *, uint32_t> first_owner;
         for (uint32_t j = 0; j < n_jobs; ++j) {
@@ -894,10 +898,10 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         }
         const double budget = kBinFill * (double)((size_t)1 << ctx->tuning.bin_log2);
         double used = 0.0, most = 0.0;
-        uint32_t r = 0;
+        uint32_t r = 1;
         for (size_t k = 0; k < items.size(); ++k) {
             const spx_job &jb = jobs[items[k].job];
-            if (jb.kind == SPX_JOB_CAMERA) continue;  // round 0
+            if (jb.kind == SPX_JOB_CAMERA) continue;
             double est = (double)items[k].count * records_per_photon(jb.kind == SPX_JOB_LIGHTTRACE);
             if (used > 0.0 && used + est > budget) {
                 ++r;
@@ -1008,7 +1012,11 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     ctx->batch_pending = true;
     const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
     for (uint32_t r = 0; r < n_rounds && e == hipSuccess; ++r) {
-        if (binned) e = launch_bins_reset(bst, s);
+        uint32_t round_items = 0;
+        for (uint32_t q = 0; q < kXcdQueues; ++q) round_items += qh[r].size[q];
+        if (round_items == 0) continue;
+        const bool camera = r == 0;
+        if (binned && !camera) e = launch_bins_reset(bst, s);
         hipEvent_t ev_end = nullptr;
         if (e == hipSuccess && ctx->ktime_on) {
             while (ctx->ktime_ev.size() < ctx->ktime_used + 2) {
@@ -1021,13 +1029,13 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
             if (e == hipSuccess) ev_end = ctx->ktime_ev[ctx->ktime_used + 1];
         }
         if (e == hipSuccess)
-            e = launch_batch((const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items),
+            e = launch_batch(camera, (const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items),
                              (BatchQueues *)(d + r * qstride), blocks, splat_mode, bst, s);
         if (e == hipSuccess && ev_end) {
             e = hipEventRecord(ev_end, s);
             if (e == hipSuccess) ctx->ktime_used += 2;
         }
-        if (e == hipSuccess && binned) e = launch_bins_resolve(bst, job_shared, acc_blocks, s);
+        if (e == hipSuccess && binned && !camera) e = launch_bins_resolve(bst, job_shared, acc_blocks, s);
     }
     if (e == hipSuccess && !reps.empty())
         e = launch_reduce_replicas((const ReplicaJob *)(d + off_reps), (uint32_t)reps.size(), rep_blocks, s);

@@ -28,7 +28,9 @@ struct BatchQueues {
     uint32_t pad[8];
 };
 
-hipError_t launch_batch(const BatchJob *jobs, const BatchItem *items, BatchQueues *q, uint32_t blocks,
-                        uint32_t splat_mode, const BinState &bins, hipStream_t s);
+// camera: the camera kernel (items of camera jobs only); else the photon
+// kernel for splat_mode (items of volume / lighttrace jobs only).
+hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *items, BatchQueues *q,
+                        uint32_t blocks, uint32_t splat_mode, const BinState &bins, hipStream_t s);
 
 }  // namespace spx
