@@ -2,17 +2,22 @@
 
 One step = the mixed stream of 512 uneven jobs (half rainbow camera renders,
 half prism-on-floor volume renders; spectral_amd.jobs.mixed_jobs, seed 0)
-traced by the batched engine in one persistent launch, inputs resident in
-HBM.  value = (paths + photons) per second / 1e6 over exactly K timed steps.
+traced by the batched engine, inputs resident in HBM: the table's schedule is
+captured once as a hipGraph (spx_batch_plan_create) and each step replays it.
+The launch shape is picked first by the autotuner (spx_autotune) on every
+fourth job of the stream.  value = (paths + photons) per second / 1e6 over
+exactly K timed steps.
 
 Also reported on the same line: the plain one-job-per-launch form on the same
-stream, the per-configuration throughputs (DSOTM prism 2^20 photons at
+stream, per-kernel times of the batched step (HIP events around each kernel
+group of K eager steps on the launch stream) and the roofline of the dominant
+kernel, the per-configuration throughputs (DSOTM prism 2^20 photons at
 320x180; rainbow 512^2 x 64 spp; full-size prism-on-floor 2^28 photons at
 1920x1080 under D65, A and dispersion off, plus its floor caustic), the
-served stream's p50/p99 job latency, the roofline of the batched kernel and
-the CPU oracle timed on this host's cores.
+served stream's p50/p99 job latency and the CPU oracle timed on this host.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+                       [--no-tune | --tuning JSON] [--no-cpu] [--no-configs] [--no-serve] [--no-plain]
 """
 from __future__ import annotations
 
@@ -36,6 +41,13 @@ HBM_PEAK_GBS = 8000.0
 # photon segment moves a 40 B state in and out (80 B) and each film deposit
 # is 3 fp32 atomics (12 B).
 CAM_SEG_BYTES, CAM_PATH_BYTES, PH_SEG_BYTES, SPLAT_BYTES = 88, 12, 80, 12
+# Segments per camera path, segments and film deposits per photon of the
+# mixed stream's two job classes, as the oracle counts them on the samples
+# cpu_baseline runs (used only when that leg is skipped; the CPU test
+# tests/test_work_model.py re-derives them from the oracle).
+WORK_MODEL_PINNED = {"seg_per_path": 1.787, "seg_per_photon": 4.773, "splat_per_photon": 13.94}
+# Volume samples of the CPU leg: (prism rotation deg, beam offset).
+VOLUME_SAMPLES = ((0.0, 0.0), (20.0, 0.05), (45.0, -0.08))
 
 
 def log(*a):
@@ -59,36 +71,15 @@ class Timer:
         return ms / reps, (time.perf_counter() - t0) * 1e3 / reps
 
 
-def work_model():
-    """Segments per path / photon and deposits per photon, counted by the
-    oracle on samples of the two job classes."""
-    from oracle import oracle as O
-    from spectral_amd import _abi as A
-    from spectral_amd import jobs, scenes
-    cam_spec = jobs.JobSpec(A.JOB_CAMERA, 96, spp=4, seed=1)
-    sc, cam, _ = jobs.job_geometry(cam_spec)
-    _, st = O.camera_accumulate(sc, cam, 96, 96, 4, 0, 1, jobs.job_opts(cam_spec), threads=8)
-    seg_per_path = st.segments / st.paths
-    segs = splats = n = 0
-    for rot, off in ((0.0, 0.0), (20.0, 0.05), (45.0, -0.08)):
-        sp = jobs.JobSpec(A.JOB_VOLUME, 256, n_photons=1 << 13, rotation=rot, offset=off, seed=2)
-        sc, cam, bm = jobs.job_geometry(sp)
-        _, s2 = O.volume(sc, cam, bm, 256, 256, sp.n_photons, 0, A.VolParams(**scenes.HAZE), None, 2,
-                         A.EST_POINT, jobs.job_opts(sp), threads=8)
-        segs, splats, n = segs + s2.segments, splats + s2.splats, n + s2.paths
-    return {"seg_per_path": seg_per_path, "seg_per_photon": segs / n, "splat_per_photon": splats / n}
-
-
-def profiled_traffic():
-    """HBM bytes per k_batch launch on this workload, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "k_batch_mixed512_hbm_counters.md")
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or
+    None when the summary has no figure for it."""
+    path = ROOT / "profiles" / "batched_step_mixed512_hbm_counters.md"
     try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
-                    return float(line.split(":", 1)[1])
+        for line in path.read_text().splitlines():
+            if line.startswith(f"traffic_bytes_per_launch[{kernel}]:"):
+                return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
@@ -96,7 +87,9 @@ def profiled_traffic():
 
 def cpu_baseline(paths_total, photons_total, target_s=12.0):
     """The oracle on this host: camera paths on all cores (the reference's
-    rayon loop), volume photons single-threaded (volume.rs:118)."""
+    rayon loop), volume photons single-threaded (volume.rs:118), on bounded
+    samples of the two job classes.  Also returns the oracle's segment and
+    deposit counts on those samples (the work model of the roofline)."""
     from oracle import oracle as O
     from spectral_amd import _abi as A
     from spectral_amd import jobs, scenes
@@ -109,28 +102,36 @@ def cpu_baseline(paths_total, photons_total, target_s=12.0):
     spp = max(1, int(2 * (target_s / 2) / max(t_cam, 1e-3)))
     spp = min(spp, 64)
     t0 = time.perf_counter()
-    O.camera_accumulate(sc, cam, 256, 256, spp, 2, 1, jobs.job_opts(spec), threads=cores)
+    _, cst = O.camera_accumulate(sc, cam, 256, 256, spp, 2, 1, jobs.job_opts(spec), threads=cores)
     t_cam = time.perf_counter() - t0
     cam_rate = 256 * 256 * spp / t_cam
-    sp = jobs.JobSpec(A.JOB_VOLUME, 512, n_photons=1 << 14, rotation=20.0, seed=2)
-    sc, cam, bm = jobs.job_geometry(sp)
     vp = A.VolParams(**scenes.HAZE)
-    t0 = time.perf_counter()
-    O.volume(sc, cam, bm, 512, 512, sp.n_photons, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp), threads=1)
-    t1 = time.perf_counter() - t0
-    n = min(1 << 22, max(1 << 14, int(sp.n_photons * (target_s / 2) / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    O.volume(sc, cam, bm, 512, 512, n, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp), threads=1)
-    t_ph = time.perf_counter() - t0
-    ph_rate = n / t_ph
+
+    def volume_samples(n):
+        segs = splats = paths = 0
+        t0 = time.perf_counter()
+        for rot, off in VOLUME_SAMPLES:
+            sp = jobs.JobSpec(A.JOB_VOLUME, 512, n_photons=n, rotation=rot, offset=off, seed=2)
+            sc, cam, bm = jobs.job_geometry(sp)
+            _, st = O.volume(sc, cam, bm, 512, 512, n, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp),
+                             threads=1)
+            segs, splats, paths = segs + st.segments, splats + st.splats, paths + st.paths
+        return time.perf_counter() - t0, segs, splats, paths
+
+    t1, *_ = volume_samples(1 << 12)
+    n = min(1 << 21, max(1 << 12, int((1 << 12) * (target_s / 2) / max(t1, 1e-3))))
+    t_ph, segs, splats, nph = volume_samples(n)
+    ph_rate = nph / t_ph
     step_s = paths_total / cam_rate + photons_total / ph_rate
+    work = {"seg_per_path": cst.segments / cst.paths, "seg_per_photon": segs / nph,
+            "splat_per_photon": splats / nph}
     return {"value": round((paths_total + photons_total) / step_s / 1e6, 3), "unit": UNIT,
             "cores": cores, "kind": "port",
             "sample": (f"oracle (C restatement of spectral-core): rainbow 256x256x{spp} spp on {cores} "
-                       f"threads = {cam_rate / 1e6:.3f} Mpaths/s; prism-on-floor volume {n} photons "
-                       f"at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; value = the mixed "
-                       f"step's paths and photons at those rates"),
-            "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4)}
+                       f"threads = {cam_rate / 1e6:.3f} Mpaths/s; prism-on-floor volume 3 x {n} photons "
+                       f"(3 prism rotations) at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; "
+                       f"value = the mixed step's paths and photons at those rates"),
+            "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4)}, work
 
 
 def per_config(ctx, timer, quick):
@@ -183,12 +184,26 @@ def per_config(ctx, timer, quick):
     return out
 
 
+def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
+    """Algorithmic HBM bytes of a kernel group over its measured time."""
+    per_launch = alg_bytes_step / max(launches_step, 1)
+    ms_launch = ms_step / max(launches_step, 1)
+    achieved = per_launch / (ms_launch / 1e3) / 1e9 if ms_launch > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(name), "kernel": name,
+            "kernel_ms_per_launch": round(ms_launch, 3), "launches_per_step": launches_step,
+            "alg_bytes_per_launch": round(per_launch)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--quick", action="store_true", help="smaller per-config sizes")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (no autotuner)")
+    ap.add_argument("--tuning", type=str, default=None,
+                    help="JSON object of spx_tuning fields to install instead of autotuning")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -222,18 +237,40 @@ def main() -> int:
     total = paths + photons
     log(f"mixed stream: {len(specs)} jobs, {paths / 1e9:.3f} G paths, {photons / 1e9:.3f} G photons")
     br = jobs.BatchRunner(ctx, specs)
-    t = ctx.tuning()
-    cfg = A.BatchCfg(t.splat_mode, 0, 0, 0)
+
+    # launch shape: explicit, default, or autotuned on every fourth job
+    tune_info = {"source": "default"}
+    if args.tuning:
+        ctx.set_tuning(**json.loads(args.tuning))
+        tune_info = {"source": "given"}
+    elif not args.no_tune:
+        t0 = time.perf_counter()
+        _, tune_ms = br.autotune(stream, sweeps=1, subset=list(range(0, len(specs), 4)))
+        tune_info = {"source": "spx_autotune on every 4th job", "seconds": round(time.perf_counter() - t0, 1),
+                     "sample_ms": round(tune_ms, 3)}
+    tuning = ctx.tuning().as_dict()
+    log(f"tuning ({tune_info['source']}): {tuning}")
+
+    # the timed region: K replays of the captured schedule
+    t0 = time.perf_counter()
+    plan = br.plan(stream)
+    tune_info["plan"] = dict(plan.info(), capture_ms=round((time.perf_counter() - t0) * 1e3, 1))
     for _ in range(args.warmup):
-        br.run_batched(stream, cfg)
+        plan.launch(stream)
+    ctx.sync(stream)
+    ms_step, wall_ms = timer.run(lambda: plan.launch(stream), args.steps)
+    value = total / ms_step / 1e3
+    log(f"batched (graph replay): {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
+    plan.close()
+    # per-kernel times: the same schedule run eagerly, each kernel group
+    # bracketed by HIP events on the launch stream
+    br.run_batched(stream)
     ctx.sync(stream)
     ctx.kernel_timing(True)
-    ms_step, wall_ms = timer.run(lambda: br.run_batched(stream, cfg), args.steps)
-    k_ms, k_launches = ctx.kernel_time()
+    eager_ms, _ = timer.run(lambda: br.run_batched(stream), args.steps)
+    kt = ctx.kernel_times()
     ctx.kernel_timing(False)
-    k_ms_step = k_ms / args.steps  # k_batch time per step (one launch per deposit round)
-    value = total / ms_step / 1e3
-    log(f"batched: {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
+    log(f"batched (eager): {eager_ms:.2f} ms/step; kernel groups {kt}")
     if args.no_plain:
         ms_plain = float("nan")
     else:
@@ -242,38 +279,46 @@ def main() -> int:
     log(f"plain one-job-per-launch: {ms_plain:.2f} ms -> {plain_value:.1f}")
     br.close()
 
-    wm = work_model()
-    alg_bytes = paths * (wm["seg_per_path"] * CAM_SEG_BYTES + CAM_PATH_BYTES) + photons * (
-        wm["seg_per_photon"] * PH_SEG_BYTES + wm["splat_per_photon"] * SPLAT_BYTES)
-    achieved = alg_bytes / (k_ms_step / 1e3) / 1e9
-    traffic = profiled_traffic()
+    cpu, wm = (None, dict(WORK_MODEL_PINNED)) if args.no_cpu else cpu_baseline(paths, photons)
+    cam_bytes = paths * (wm["seg_per_path"] * CAM_SEG_BYTES + CAM_PATH_BYTES)
+    ph_bytes = photons * (wm["seg_per_photon"] * PH_SEG_BYTES + wm["splat_per_photon"] * SPLAT_BYTES)
+    dep_bytes = photons * wm["splat_per_photon"] * SPLAT_BYTES
+    steps = args.steps
+    groups = {
+        "camera": kernel_roofline("k_batch_camera", cam_bytes, kt["camera"][0] / steps, kt["camera"][1] // steps),
+        "photon": kernel_roofline(f"k_batch_photons<{tuning['splat_mode']}>", ph_bytes, kt["photon"][0] / steps,
+                                  kt["photon"][1] // steps),
+    }
+    if kt["bins"][1]:
+        groups["bins"] = kernel_roofline("k_bins passes", dep_bytes, kt["bins"][0] / steps, kt["bins"][1] // steps)
+    dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
+    roofline = dict(groups[dominant])
+    roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
+                                                 "frac", "traffic")} for g, v in groups.items()}
+    roofline["work_model"] = {k: round(v, 4) for k, v in wm.items()}
+    roofline["work_model_source"] = "oracle counts on the cpu_baseline samples" if cpu else "pinned constants"
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": UNIT, "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "512 mixed jobs: rainbow camera renders + prism-on-floor volume renders",
-                   "jobs": len(specs), "paths": paths, "photons": photons, "engine": "batched",
-                   "generator_seed": 0},
+                   "jobs": len(specs), "paths": paths, "photons": photons,
+                   "engine": "batched, captured hipGraph replay", "generator_seed": 0},
         "plain_value": None if args.no_plain else round(plain_value, 2),
         "batched_over_plain": None if args.no_plain else round(value / plain_value, 3),
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "k_batch", "kernel_ms_per_launch": round(k_ms / max(k_launches, 1), 3),
-                     "launches_per_step": k_launches // args.steps, "alg_bytes": round(alg_bytes),
-                     "work_model": {k: round(v, 4) for k, v in wm.items()}},
+        "eager_batched_ms_per_step": round(eager_ms, 3),
+        "tuning": dict(tuning, **tune_info),
+        "roofline": roofline,
     }
     if not args.no_configs:
         result["configs"] = per_config(ctx, timer, args.quick)
         log(json.dumps(result["configs"]))
     if not args.no_serve:
-        try:
-            from spectral_amd import serve
-            result["serve"] = serve.bench_serve(ctx, quick=args.quick)
-        except ImportError:
-            result["serve"] = None
-    if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(paths, photons)
+        from spectral_amd import serve
+        result["serve"] = serve.bench_serve(ctx, quick=args.quick)
+    if cpu is not None:
+        result["cpu_baseline"] = cpu
     print(json.dumps(result), flush=True)
     ctx.close()
     return 0

@@ -155,16 +155,24 @@ typedef struct spx_ctx spx_ctx;
 typedef struct spx_scene spx_scene;
 
 /* Per-context tuning of the wavefront and batched engines (the plain engine
- * ignores it).  spx_autotune picks these values. */
+ * ignores it).  spx_autotune picks these values by timing a job table. */
 typedef struct {
     uint32_t max_replicas;   /* photon film copies per job (1 = deposit straight into the film) */
     uint32_t replica_mib;    /* per-job budget for the copies, MiB */
     uint32_t splat_mode;     /* photon deposits: 0 = direct fp32 atomics, 1 = LDS hash first,
-                                2 = binned records sorted by 32x32 tile and summed in LDS as
-                                fixed point (no fp32 atomics; films bit-reproducible); default 2 */
-    uint32_t chunk_log2;     /* wavefront chunk: 2^chunk_log2 paths / photons */
+                                2 = binned records sorted by film tile and summed in LDS as
+                                fixed point (no fp32 atomics; films bit-reproducible), 3 = the
+                                LDS hash draining to binned records (fewer records; fp32 sums
+                                in LDS, so not bit-reproducible); default 2 */
+    uint32_t chunk_log2;     /* wavefront queue capacity: 2^chunk_log2 paths / photons per bounce */
     uint32_t bin_log2;       /* binned deposits: at most 2^bin_log2 records per round (32 B of
                                 device memory each; buffers sized to the work); default 30 */
+    uint32_t tile_wlog2;     /* binned deposits: film tiles of 1024 pixels, 2^tile_wlog2 wide and
+                                2^(10 - tile_wlog2) tall (2..8); default 5 = 32x32 */
+    uint32_t batch_blocks;   /* batched engine: persistent grid size in blocks of 256 threads
+                                (0 = 4 per compute unit) */
+    uint32_t chunk_paths;    /* batched engine: camera work item, in paths (default 16384) */
+    uint32_t chunk_photons;  /* batched engine: photon work item, in photons (default 8192) */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */
@@ -178,12 +186,20 @@ int spx_ctx_num_cus(const spx_ctx *ctx);
 int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t);
 int spx_ctx_get_tuning(const spx_ctx *ctx, spx_tuning *t);
 /* Measurement hook (no counterpart in the reference): while enabled, every
- * launch of the batched engine's trace kernels (k_batch_camera, k_batch_photons) is bracketed by HIP
- * events on the caller's stream.  spx_ctx_kernel_time waits for the launches
- * recorded so far, returns their summed duration (ms) and count, and clears
- * the record.  Enabling clears it too. */
+ * spx_batch_run brackets its kernel groups with HIP events on the caller's
+ * stream: the camera trace kernel (k_batch_camera), each photon trace
+ * launch (k_batch_photons) and each round's bin passes (count, scan,
+ * scatter, accumulate).  spx_ctx_kernel_time waits for the launches recorded
+ * so far, returns the summed duration (ms) and count of the two trace
+ * kernels, and clears the record; spx_ctx_kernel_times returns the same per
+ * group (index SPX_KT_*) and clears it.  Enabling clears it too.  Graph
+ * replays (spx_batch_plan_launch) are not bracketed. */
+#define SPX_KT_CAMERA 0u
+#define SPX_KT_PHOTON 1u
+#define SPX_KT_BINS 2u
 int spx_ctx_kernel_timing(spx_ctx *ctx, int enable);
 int spx_ctx_kernel_time(spx_ctx *ctx, double *total_ms, uint32_t *launches);
+int spx_ctx_kernel_times(spx_ctx *ctx, double ms[3], uint32_t launches[3]);
 
 /* Camera::look_at (camera.rs:15-27), evaluated on the host.  Needs no device. */
 int spx_camera_look_at(const float origin[3], const float target[3], const float up[3],
@@ -282,9 +298,44 @@ typedef struct {
 /* Runs every job of the table in ONE persistent launch on `stream`.  Work
  * items are (job, range) chunks taken from a device queue; results equal
  * the per-job calls at the same seeds and bases within float summation
- * order. */
+ * order.  cfg NULL = the context's tuning. */
 int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg,
                   void *stream);
+
+/* A batch plan: the launch schedule of one job table -- device job table,
+ * work items per XCD queue, the camera round and the photon deposit rounds
+ * with their record, replica and queue buffers, all owned by the plan --
+ * captured once as a hipGraph (queue reset, trace kernel and bin passes of
+ * every round) and replayed by spx_batch_plan_launch.  No counterpart in the
+ * reference (its GPU mirror records one command buffer per dispatch).
+ * Creation uploads on `stream` and waits for it. */
+typedef struct spx_batch_plan spx_batch_plan;
+int spx_batch_plan_create(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg,
+                          void *stream, spx_batch_plan **out);
+/* Rewrites the per-job values that leave the schedule unchanged -- RNG
+ * bases, seeds, cameras, beams, screen geometry, volume weights, options,
+ * scenes, film and z-buffer pointers -- with a stream-ordered upload on
+ * `stream`.  Kinds, film sizes, spp, photon counts, estimators and which jobs
+ * share a film must equal the plan's (SPX_EINVAL otherwise: create a new
+ * plan). */
+int spx_batch_plan_update(spx_batch_plan *plan, const spx_job *jobs, uint32_t n_jobs, void *stream);
+/* Replays the captured graph on `stream` (ordered after earlier work on it). */
+int spx_batch_plan_launch(spx_batch_plan *plan, void *stream);
+/* Rounds in the plan's schedule (camera round included) and graph nodes. */
+int spx_batch_plan_info(const spx_batch_plan *plan, uint32_t *rounds, uint32_t *graph_nodes);
+void spx_batch_plan_destroy(spx_batch_plan *plan);
+
+/* Autotuner of the batched engine (no counterpart in the reference).  Times
+ * the job table under candidate tunings -- persistent grid size, camera and
+ * photon work-item sizes, deposit mode, record capacity per round and film
+ * tile shape -- one coordinate at a time for `sweeps` passes (1..4) starting
+ * from the context's tuning.  Each candidate runs the table into scratch
+ * films (the jobs' own films are not touched) once untimed and twice timed
+ * with HIP events on `stream`; the faster timed run counts.  The best tuning
+ * is installed in the context and returned in *out; *best_ms (optional) is
+ * its time per table.  Synchronous. */
+int spx_autotune(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, uint32_t sweeps, void *stream,
+                 spx_tuning *out, double *best_ms);
 
 /* ---- serving: continuous batching over resident films ------------------------- */
 /* A request is K frames of one job.  Frame f traces the job with its RNG
@@ -309,7 +360,7 @@ typedef struct {
 typedef struct {
     uint32_t state;       /* SPX_REQ_* */
     uint32_t frames_done;
-    uint64_t footprint;   /* film + deposit-replica bytes reserved at admission */
+    uint64_t footprint;   /* film + deposit-queue bytes reserved at admission */
     double t_submit;      /* seconds on the server clock */
     double t_admit;
     double t_done;
@@ -335,6 +386,11 @@ int spx_server_film(spx_server *srv, uint64_t id, float *host_out, size_t floats
 int spx_server_release(spx_server *srv, uint64_t id);
 uint64_t spx_server_bytes_in_use(const spx_server *srv);
 double spx_server_clock(const spx_server *srv);
+/* Steps replay a captured batch plan while the active set is unchanged
+ * (default 1); 0 = every step is one spx_batch_run. */
+int spx_server_set_graphs(spx_server *srv, int enable);
+/* Steps that captured a new plan and steps that replayed one. */
+int spx_server_stats(const spx_server *srv, uint64_t *captures, uint64_t *replays);
 
 /* ---- device memory and streams (so callers need no HIP bindings) --------------- */
 int spx_malloc(spx_ctx *ctx, size_t bytes, void **out);

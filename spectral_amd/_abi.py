@@ -26,6 +26,7 @@ ILLUM_D65, ILLUM_A, ILLUM_E = 0, 1, 2
 EST_POINT, EST_BEAM, EST_LEGACY = 0, 1, 2
 ENGINE_PLAIN, ENGINE_WAVEFRONT = 0, 1
 JOB_CAMERA, JOB_VOLUME, JOB_LIGHTTRACE = 0, 1, 2
+KT_CAMERA, KT_PHOTON, KT_BINS = 0, 1, 2
 
 F3 = C.c_float * 3
 
@@ -88,7 +89,12 @@ class Job(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("max_replicas", C.c_uint32), ("replica_mib", C.c_uint32),
-                ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32), ("bin_log2", C.c_uint32)]
+                ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32), ("bin_log2", C.c_uint32),
+                ("tile_wlog2", C.c_uint32), ("batch_blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
+                ("chunk_photons", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
 class BatchCfg(C.Structure):
@@ -124,6 +130,7 @@ SIGNATURES = {
     "spx_ctx_get_tuning": (_i32, [_vp, _P(Tuning)]),
     "spx_ctx_kernel_timing": (_i32, [_vp, _i32]),
     "spx_ctx_kernel_time": (_i32, [_vp, _P(C.c_double), _P(_u32)]),
+    "spx_ctx_kernel_times": (_i32, [_vp, _P(C.c_double), _P(_u32)]),
     "spx_atomic_probe": (_i32, [_vp, _vp, _sz, _u32, _u32, _vp]),
     "spx_camera_look_at": (_i32, [_P(_f32), _P(_f32), _P(_f32), _f32, _f32, _P(CameraBasis)]),
     "spx_scene_create": (_i32, [_vp, _P(Env), _P(Prim), _u32, _P(Plane), _u32, _P(Material), _u32,
@@ -142,6 +149,12 @@ SIGNATURES = {
     "spx_rng_draws": (_i32, [_vp, _u32, _u32, _u32, _P(_u32)]),
     "spx_glass_n": (_i32, [_vp, _u32, _P(_f32), _u32, _P(_f32)]),
     "spx_batch_run": (_i32, [_vp, _P(Job), _u32, _P(BatchCfg), _vp]),
+    "spx_batch_plan_create": (_i32, [_vp, _P(Job), _u32, _P(BatchCfg), _vp, _P(_vp)]),
+    "spx_batch_plan_update": (_i32, [_vp, _P(Job), _u32, _vp]),
+    "spx_batch_plan_launch": (_i32, [_vp, _vp]),
+    "spx_batch_plan_info": (_i32, [_vp, _P(_u32), _P(_u32)]),
+    "spx_batch_plan_destroy": (None, [_vp]),
+    "spx_autotune": (_i32, [_vp, _P(Job), _u32, _u32, _vp, _P(Tuning), _P(C.c_double)]),
     "spx_server_create": (_i32, [_vp, C.c_uint64, _P(_vp)]),
     "spx_server_destroy": (None, [_vp]),
     "spx_server_submit": (_i32, [_vp, _P(Request), _P(C.c_uint64)]),
@@ -152,6 +165,8 @@ SIGNATURES = {
     "spx_server_release": (_i32, [_vp, C.c_uint64]),
     "spx_server_bytes_in_use": (C.c_uint64, [_vp]),
     "spx_server_clock": (C.c_double, [_vp]),
+    "spx_server_set_graphs": (_i32, [_vp, _i32]),
+    "spx_server_stats": (_i32, [_vp, _P(C.c_uint64), _P(C.c_uint64)]),
     "spx_malloc": (_i32, [_vp, _sz, _P(_vp)]),
     "spx_free": (_i32, [_vp, _vp]),
     "spx_memcpy_h2d": (_i32, [_vp, _vp, _vp, _sz, _vp]),

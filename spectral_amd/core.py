@@ -62,6 +62,14 @@ class Context:
         _check(self, self._lib.spx_ctx_kernel_time(self.handle, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
+    def kernel_times(self) -> dict:
+        """{group: (summed ms, launches)} for the camera trace kernel, the photon
+        trace kernel and the bin passes since the last read."""
+        ms, n = (C.c_double * 3)(), (C.c_uint32 * 3)()
+        _check(self, self._lib.spx_ctx_kernel_times(self.handle, ms, n))
+        return {name: (ms[k], n[k]) for k, name in ((A.KT_CAMERA, "camera"), (A.KT_PHOTON, "photon"),
+                                                    (A.KT_BINS, "bins"))}
+
     def close(self) -> None:
         if self.handle:
             self._lib.spx_ctx_destroy(self.handle)

@@ -152,14 +152,15 @@ __global__ __launch_bounds__(256, 4) void k_batch_camera(const BatchJob *__restr
 }
 
 // Photon items only (volume and lighttrace), deposits through sink SPLAT:
-// 0 = direct film atomics, 1 = LDS hash, 2 = binned records.
+// 0 = direct film atomics, 1 = LDS hash, 2 = binned records, 3 = LDS hash
+// draining to binned records.
 template <int SPLAT>
 __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restrict__ jobs,
                                                        const BatchItem *__restrict__ items, BatchQueues *q,
                                                        BinState bins) {
     __shared__ BatchBlock B;
-    __shared__ uint32_t sk[SPLAT == 1 ? kBatchSlots : 1];
-    __shared__ float sv[SPLAT == 1 ? 3 * kBatchSlots : 1];
+    __shared__ uint32_t sk[SPLAT == 1 || SPLAT == 3 ? kBatchSlots : 1];
+    __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kBatchSlots : 1];
     __shared__ uint32_t sfill;
     __shared__ uint32_t wstate[2 * 4];  // binned deposits: {chunk, fill} per wave
     LdsSplat<kBatchSlots> hash{sk, sv, &sfill, nullptr};
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
         B.scene = nullptr;
         B.tbl_src = nullptr;
     }
-    if (SPLAT == 1) hash.init();
+    if (SPLAT == 1 || SPLAT == 3) hash.init();
     if (threadIdx.x < 8) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
     const uint32_t pool = xcc_id() % kBinPools;
     const uint32_t home = blockIdx.x % kXcdQueues;
@@ -184,9 +185,17 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
             GlobalSink g{deposit_film(P)};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
         } else if (SPLAT == 2) {
+            const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
             BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
-                      wstate, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w};
+                      wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
+        } else if (SPLAT == 3) {
+            const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
+            HashBinSink<kBatchSlots> hb{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
+                                                      pool, P.key_base, P.film, wstate, fw,
+                                                      tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
+            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hb);
+            hb.flush();
         } else {
             hash.film = deposit_film(P);
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hash);
@@ -194,7 +203,22 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
         }
         __syncthreads();
     }
-    if (SPLAT == 2) bin_wave_finish(wstate, bins.chunk_fill);
+    if (SPLAT >= 2) bin_wave_finish(wstate, bins.chunk_fill);
+}
+
+// Zeroes the claim counters of every round's queues (the graph replays the
+// same image, so the counters are reset by a kernel, not by the upload).
+__global__ __launch_bounds__(256) void k_queues_reset(char *base, uint32_t n_rounds, uint32_t stride) {
+    for (uint32_t i = threadIdx.x; i < n_rounds * kXcdQueues; i += blockDim.x) {
+        BatchQueues *q = reinterpret_cast<BatchQueues *>(base + (size_t)(i / kXcdQueues) * stride);
+        q->next[i % kXcdQueues] = 0u;
+    }
+}
+
+hipError_t launch_queues_reset(BatchQueues *q, uint32_t n_rounds, size_t stride, hipStream_t s) {
+    hipLaunchKernelGGL(k_queues_reset, dim3(1), dim3(256), 0, s, reinterpret_cast<char *>(q), n_rounds,
+                       (uint32_t)stride);
+    return hipGetLastError();
 }
 
 hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *items, BatchQueues *q,
@@ -205,8 +229,10 @@ hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *item
         hipLaunchKernelGGL(k_batch_photons<0>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
     else if (splat_mode == 1)
         hipLaunchKernelGGL(k_batch_photons<1>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
-    else
+    else if (splat_mode == 2)
         hipLaunchKernelGGL(k_batch_photons<2>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+    else
+        hipLaunchKernelGGL(k_batch_photons<3>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
     return hipGetLastError();
 }
 

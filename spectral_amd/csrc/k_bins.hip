@@ -415,8 +415,7 @@ __global__ __launch_bounds__(256) void k_bins_accumulate(BinState st, const uint
         const uint32_t tile = ((s << kStripLog2) - st.job_key[job]) >> kStripLog2;
         const uint32_t npix = st.job_npix[job], width = st.job_width[job];
         const uint32_t height = npix / width;
-        const uint32_t tiles_x = (width + (1u << kTileLog2) - 1) >> kTileLog2;
-        const uint32_t tx0 = (tile % tiles_x) << kTileLog2, ty0 = (tile / tiles_x) << kTileLog2;
+        const uint32_t tiles_x = tiles_across(width, st.tile_wlog2);
         float *film = st.job_film[job];
         const bool exclusive = nparts == 1 && !job_shared[job];
         if (fixed && nparts > 1) {  // into the strip's accumulator; written by k_bins_macc_flush
@@ -427,7 +426,8 @@ __global__ __launch_bounds__(256) void k_bins_accumulate(BinState st, const uint
             continue;
         }
         for (uint32_t k = threadIdx.x; k < kStripPix; k += blockDim.x) {
-            uint32_t px = tx0 + (k & ((1u << kTileLog2) - 1)), py = ty0 + (k >> kTileLog2);
+            uint32_t px, py;
+            tile_pixel(tile, k, tiles_x, st.tile_wlog2, px, py);
             float x, y, z;
             if (fixed) {
                 x = from_fixed(acc[3 * k], e);
@@ -464,9 +464,8 @@ __global__ __launch_bounds__(256) void k_bins_macc_flush(BinState st, const uint
         const uint32_t job = st.strip_job[s];
         const uint32_t tile = ((s << kStripLog2) - st.job_key[job]) >> kStripLog2;
         const uint32_t npix = st.job_npix[job], width = st.job_width[job];
-        const uint32_t tiles_x = (width + (1u << kTileLog2) - 1) >> kTileLog2;
-        const uint32_t px = ((tile % tiles_x) << kTileLog2) + (k & ((1u << kTileLog2) - 1));
-        const uint32_t py = ((tile / tiles_x) << kTileLog2) + (k >> kTileLog2);
+        uint32_t px, py;
+        tile_pixel(tile, k, tiles_across(width, st.tile_wlog2), st.tile_wlog2, px, py);
         if (px >= width || py >= npix / width) continue;
         const float x = from_fixed(a[0], e), y = from_fixed(a[1], e), z = from_fixed(a[2], e);
         float *film = st.job_film[job];

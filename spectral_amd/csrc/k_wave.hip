@@ -237,7 +237,8 @@ __device__ inline bool ph_bounce(const SceneData &S, const float4 *tbl, const Ph
     return volume_bounce<SPX_EST_POINT>(S, tbl, P, st, sink);
 }
 
-// Per-kernel deposit sink: SINK 1 = LDS hash + atomics, 2 = binned records.
+// Per-kernel deposit sink: SINK 1 = LDS hash + atomics, 2 = binned records,
+// 3 = LDS hash draining to binned records.
 template <int SINK>
 struct PhSink;
 
@@ -250,6 +251,14 @@ struct PhSink<1> {
 
 template <>
 struct PhSink<2> {
+    uint32_t wstate[2 * 4];
+};
+
+template <>
+struct PhSink<3> {
+    uint32_t sk[kSplatSlots];
+    float sv[3 * kSplatSlots];
+    uint32_t sfill;
     uint32_t wstate[2 * 4];
 };
 
@@ -274,7 +283,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
                 uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
                 if (alive) ph_store(qout, at, st);
             }
-            if constexpr (SINK == 1) sink.maybe_flush();
+            if constexpr (SINK == 1 || SINK == 3) sink.maybe_flush();
         }
     };
     if constexpr (SINK == 1) {
@@ -283,11 +292,24 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
         __syncthreads();
         step(sink);
         sink.flush();
+    } else if constexpr (SINK == 3) {
+        LdsSplat<kSplatSlots> hash{L.sk, L.sv, &L.sfill, nullptr};
+        hash.init();
+        if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
+        __syncthreads();
+        const uint32_t fw = MODE == 3 ? P.scr_w : P.w;
+        HashBinSink<kSplatSlots> sink{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
+                                                    xcc_id() % kBinPools, P.key_base, P.film, L.wstate, fw,
+                                                    tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
+        step(sink);
+        sink.flush();
+        bin_wave_finish(L.wstate, bins.chunk_fill);
     } else {
         if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
         __syncthreads();
+        const uint32_t fw = MODE == 3 ? P.scr_w : P.w;
         BinSink sink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, xcc_id() % kBinPools, P.key_base,
-                     P.film, L.wstate, MODE == 3 ? P.scr_w : P.w};
+                     P.film, L.wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
         step(sink);
         bin_wave_finish(L.wstate, bins.chunk_fill);
     }
@@ -336,19 +358,23 @@ static void run_ph(spx_ctx *ctx, const PhotonParams &p, PhWave W, uint32_t chunk
     for (uint32_t start = 0; start < p.n; start += chunk) {
         uint32_t count = p.n - start < chunk ? p.n - start : chunk;
         hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
-        if (SINK == 2) launch_bins_reset(B, s);
+        if (SINK >= 2) launch_bins_reset(B, s);
         uint32_t g = persistent_blocks(ctx, count, 4);
         for (uint32_t b = 0; b < p.max_depth; ++b)
             hipLaunchKernelGGL((k_ph<MODE, SINK>), dim3(g), dim3(256), 0, s, p, W, b, start, count, B);
-        if (SINK == 2) launch_bins_resolve(B, job_shared, (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4, s);
+        if (SINK >= 2) launch_bins_resolve(B, job_shared, (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4, s);
     }
 }
 
 hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
                                 uint32_t chunk, hipStream_t s, const BinState *bins,
-                                const uint32_t *job_shared) {
+                                const uint32_t *job_shared, bool hash) {
     PhWave W = carve_ph(scratch, chunk);
-    if (bins) {
+    if (bins && hash) {
+        if (lighttrace) run_ph<3, 3>(ctx, p, W, chunk, s, bins, job_shared);
+        else if (p.estimator == SPX_EST_LEGACY) run_ph<2, 3>(ctx, p, W, chunk, s, bins, job_shared);
+        else run_ph<0, 3>(ctx, p, W, chunk, s, bins, job_shared);
+    } else if (bins) {
         if (lighttrace) run_ph<3, 2>(ctx, p, W, chunk, s, bins, job_shared);
         else if (p.estimator == SPX_EST_LEGACY) run_ph<2, 2>(ctx, p, W, chunk, s, bins, job_shared);
         else run_ph<0, 2>(ctx, p, W, chunk, s, bins, job_shared);

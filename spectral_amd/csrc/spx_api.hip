@@ -75,6 +75,7 @@ int ensure_replicas(spx_ctx *ctx, size_t floats, hipStream_t stream) {
     (void)stream;
     size_t want = floats + floats / 4;
     hipError_t e = hipMalloc(&ctx->replica, want * sizeof(float));
+    ctx->buf_gen++;
     if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(replicas)");
     ctx->replica_floats = want;
     return SPX_OK;
@@ -116,85 +117,119 @@ int upload_async(spx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStr
     return SPX_OK;
 }
 
-static size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
-                 const uint32_t **job_shared) {
+BinLayout bins_layout(const BinJobMap &map, size_t want, uint32_t bin_log2) {
+    BinLayout L;
     // Capacity: the smallest power of two >= want, within [2^15, 2^bin_log2].
     size_t cap = (size_t)1 << 15;
-    while (cap < want && cap < ((size_t)1 << ctx->tuning.bin_log2)) cap <<= 1;
+    while (cap < want && cap < ((size_t)1 << bin_log2)) cap <<= 1;
+    L.cap = cap;
     const size_t chunks = cap / kBinChunk;
     const size_t n = map.n_strips, nj = map.film.size();
+    L.n = n;
+    L.nj = nj;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += a256(bytes); return o; };
-    size_t o_recs = take(cap * sizeof(BinRec)), o_sorted = take(cap * sizeof(BinRec));
+    L.o_recs = take(cap * sizeof(BinRec));
+    L.o_sorted = take(cap * sizeof(BinRec));
     const size_t groups = cap / kGroupRecs;
-    const size_t zero_words = n * (2 * kSub + 3);  // sub_count | sub_max | ovf_count | ovf_max | ovf_cur
-    size_t o_pool = take(kBinPools * 4), o_partn = take(4), o_fill = take(chunks * 4);
-    size_t o_zero = take(zero_words * 4), o_tot = take(n * 4), o_max = take(n * 4);
-    size_t o_off = take((n + 1) * 4), o_poff = take((n + 1) * 4);
-    size_t o_glist = take(groups * kGroupSlots * sizeof(uint2)), o_gcount = take(groups * 4);
+    L.zero_words = n * (2 * kSub + 3);  // sub_count | sub_max | ovf_count | ovf_max | ovf_cur
+    L.o_pool = take(kBinPools * 4);
+    L.o_partn = take(4);
+    L.o_fill = take(chunks * 4);
+    L.o_zero = take(L.zero_words * 4);
+    L.o_tot = take(n * 4);
+    L.o_max = take(n * 4);
+    L.o_off = take((n + 1) * 4);
+    L.o_poff = take((n + 1) * 4);
+    L.o_glist = take(groups * kGroupSlots * sizeof(uint2));
+    L.o_gcount = take(groups * 4);
     // A split strip holds more than kPartRecs records, so at most cap / kPartRecs exist.
-    const size_t max_multi = cap / kPartRecs + 1;
-    size_t o_macc = take(max_multi * 3 * kStripPix * 8), o_smacc = take(n * 4), o_maccs = take(max_multi * 4),
-           o_maccn = take(4);
-    // the uploaded map: strip_job | job_key | job_npix | job_shared | job_film
-    size_t o_map = off;
-    size_t m_sj = 0, m_key = a256(n * 4), m_npix = m_key + a256(nj * 4), m_w = m_npix + a256(nj * 4),
-           m_sh = m_w + a256(nj * 4), m_film = m_sh + a256(nj * 4), m_end = m_film + a256(nj * sizeof(float *));
-    off += m_end;
-    if (ctx->bins_bytes < off) {
-        hipError_t e = hipDeviceSynchronize();  // earlier rounds may still use the old buffers
-        if (e != hipSuccess) return check_hip(ctx, e, "prepare_bins sync");
-        if (ctx->bins) hipFree(ctx->bins);
-        ctx->bins = nullptr;
-        ctx->bins_bytes = 0;
-        size_t want = off + off / 8;
-        e = hipMalloc(&ctx->bins, want);
-        if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(bins)");
-        ctx->bins_bytes = want;
-    }
-    char *b = (char *)ctx->bins;
-    std::vector<char> host(m_end, 0);
-    memcpy(host.data() + m_sj, map.strip_job.data(), n * 4);
-    memcpy(host.data() + m_key, map.key.data(), nj * 4);
-    memcpy(host.data() + m_npix, map.npix.data(), nj * 4);
-    memcpy(host.data() + m_w, map.width.data(), nj * 4);
-    memcpy(host.data() + m_sh, map.shared.data(), nj * 4);
-    memcpy(host.data() + m_film, map.film.data(), nj * sizeof(float *));
-    int rc = upload_async(ctx, b + o_map, host.data(), m_end, s);
-    if (rc) return rc;
-    st.recs = (BinRec *)(b + o_recs);
-    st.sorted = (BinRec *)(b + o_sorted);
-    st.pool_next = (uint32_t *)(b + o_pool);
-    st.chunk_fill = (uint32_t *)(b + o_fill);
-    uint32_t *z = (uint32_t *)(b + o_zero);
+    L.max_multi = cap / kPartRecs + 1;
+    L.o_macc = take(L.max_multi * 3 * kStripPix * 8);
+    L.o_smacc = take(n * 4);
+    L.o_maccs = take(L.max_multi * 4);
+    L.o_maccn = take(4);
+    // the uploaded map: strip_job | job_key | job_npix | job_width | job_shared | job_film
+    L.o_map = off;
+    L.m_key = a256(n * 4);
+    L.m_npix = L.m_key + a256(nj * 4);
+    L.m_w = L.m_npix + a256(nj * 4);
+    L.m_sh = L.m_w + a256(nj * 4);
+    L.m_film = L.m_sh + a256(nj * 4);
+    L.m_end = L.m_film + a256(nj * sizeof(float *));
+    L.total = off + L.m_end;
+    return L;
+}
+
+void bins_bind(char *b, const BinLayout &L, const BinJobMap &map, BinState &st, const uint32_t **job_shared) {
+    const size_t n = L.n;
+    st.recs = (BinRec *)(b + L.o_recs);
+    st.sorted = (BinRec *)(b + L.o_sorted);
+    st.pool_next = (uint32_t *)(b + L.o_pool);
+    st.chunk_fill = (uint32_t *)(b + L.o_fill);
+    uint32_t *z = (uint32_t *)(b + L.o_zero);
     st.sub_count = z;
     st.sub_max = z + n * kSub;
     st.ovf_count = z + 2 * n * kSub;
     st.ovf_max = st.ovf_count + n;
     st.ovf_cur = st.ovf_max + n;
-    st.zero_words = zero_words;
-    st.strip_total = (uint32_t *)(b + o_tot);
-    st.strip_max = (uint32_t *)(b + o_max);
-    st.strip_off = (uint32_t *)(b + o_off);
-    st.part_off = (uint32_t *)(b + o_poff);
-    st.glist = (uint2 *)(b + o_glist);
-    st.gcount = (uint32_t *)(b + o_gcount);
-    st.macc = (unsigned long long *)(b + o_macc);
-    st.strip_macc = (uint32_t *)(b + o_smacc);
-    st.macc_strip = (uint32_t *)(b + o_maccs);
-    st.macc_count = (uint32_t *)(b + o_maccn);
-    st.max_multi = (uint32_t)max_multi;
-    st.part_next = (uint32_t *)(b + o_partn);
-    st.strip_job = (const uint32_t *)(b + o_map + m_sj);
-    st.job_key = (const uint32_t *)(b + o_map + m_key);
-    st.job_npix = (const uint32_t *)(b + o_map + m_npix);
-    st.job_width = (const uint32_t *)(b + o_map + m_w);
-    st.job_film = (float *const *)(b + o_map + m_film);
-    st.pool_cap = (uint32_t)(cap / kBinPools);
+    st.zero_words = L.zero_words;
+    st.strip_total = (uint32_t *)(b + L.o_tot);
+    st.strip_max = (uint32_t *)(b + L.o_max);
+    st.strip_off = (uint32_t *)(b + L.o_off);
+    st.part_off = (uint32_t *)(b + L.o_poff);
+    st.glist = (uint2 *)(b + L.o_glist);
+    st.gcount = (uint32_t *)(b + L.o_gcount);
+    st.macc = (unsigned long long *)(b + L.o_macc);
+    st.strip_macc = (uint32_t *)(b + L.o_smacc);
+    st.macc_strip = (uint32_t *)(b + L.o_maccs);
+    st.macc_count = (uint32_t *)(b + L.o_maccn);
+    st.max_multi = (uint32_t)L.max_multi;
+    st.part_next = (uint32_t *)(b + L.o_partn);
+    st.strip_job = (const uint32_t *)(b + L.o_map);
+    st.job_key = (const uint32_t *)(b + L.o_map + L.m_key);
+    st.job_npix = (const uint32_t *)(b + L.o_map + L.m_npix);
+    st.job_width = (const uint32_t *)(b + L.o_map + L.m_w);
+    st.job_film = (float *const *)(b + L.o_map + L.m_film);
+    st.pool_cap = (uint32_t)(L.cap / kBinPools);
     st.n_strips = (uint32_t)n;
-    *job_shared = (const uint32_t *)(b + o_map + m_sh);
+    st.tile_wlog2 = map.tile_wlog2;
+    *job_shared = (const uint32_t *)(b + L.o_map + L.m_sh);
+}
+
+int bins_upload_map(spx_ctx *ctx, char *b, const BinLayout &L, const BinJobMap &map, hipStream_t s) {
+    const size_t n = L.n, nj = L.nj;
+    std::vector<char> host(L.m_end, 0);
+    memcpy(host.data(), map.strip_job.data(), n * 4);
+    memcpy(host.data() + L.m_key, map.key.data(), nj * 4);
+    memcpy(host.data() + L.m_npix, map.npix.data(), nj * 4);
+    memcpy(host.data() + L.m_w, map.width.data(), nj * 4);
+    memcpy(host.data() + L.m_sh, map.shared.data(), nj * 4);
+    memcpy(host.data() + L.m_film, map.film.data(), nj * sizeof(float *));
+    return upload_async(ctx, b + L.o_map, host.data(), L.m_end, s);
+}
+
+int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
+                 const uint32_t **job_shared) {
+    const BinLayout L = bins_layout(map, want, ctx->tuning.bin_log2);
+    if (ctx->bins_bytes < L.total) {
+        hipError_t e = hipDeviceSynchronize();  // earlier rounds may still use the old buffers
+        if (e != hipSuccess) return check_hip(ctx, e, "prepare_bins sync");
+        if (ctx->bins) hipFree(ctx->bins);
+        ctx->bins = nullptr;
+        ctx->bins_bytes = 0;
+        size_t want_bytes = L.total + L.total / 8;
+        e = hipMalloc(&ctx->bins, want_bytes);
+        ctx->buf_gen++;
+        if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(bins)");
+        ctx->bins_bytes = want_bytes;
+    }
+    char *b = (char *)ctx->bins;
+    int rc = bins_upload_map(ctx, b, L, map, s);
+    if (rc) return rc;
+    bins_bind(b, L, map, st, job_shared);
     return SPX_OK;
 }
 
@@ -203,7 +238,7 @@ int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s,
 template <class F>
 static hipError_t with_bins(spx_ctx *ctx, PhotonParams &p, uint32_t w, uint32_t h, uint32_t photons, int lighttrace,
                             hipStream_t s, F launch, int *rc) {
-    BinJobMap map;
+    BinJobMap map(ctx->tuning.tile_wlog2);
     p.key_base = map.add(p.film, w, h);
     p.replicas = 1;
     p.rep_stride = 0;
@@ -278,7 +313,7 @@ static int parse_table(const char *text, int cols, float *dst /* [71][cols] */) 
     return n == kTblN ? 0 : -2;
 }
 
-static bool valid_opts(const spx_opts *o, bool camera) {
+bool spx::valid_opts(const spx_opts *o, bool camera) {
     if (o->max_depth < 1 || o->max_depth > 32) return false;
     if (o->illuminant > SPX_ILLUM_E) return false;
     if (o->dispersion > 1) return false;
@@ -286,7 +321,7 @@ static bool valid_opts(const spx_opts *o, bool camera) {
     return true;
 }
 
-static spx_opts default_opts(bool camera) {
+spx_opts spx::default_opts(bool camera) {
     spx_opts o;
     o.illuminant = camera ? SPX_ILLUM_D65 : SPX_ILLUM_E;
     o.dispersion = 1;
@@ -295,7 +330,51 @@ static spx_opts default_opts(bool camera) {
     return o;
 }
 
-static bool finite3(const float *v) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
+bool spx::finite3(const float *v) { return isfinite(v[0]) && isfinite(v[1]) && isfinite(v[2]); }
+
+int spx::check_film(spx_ctx *ctx, const float *film, uint32_t w, uint32_t h, const char *who) {
+    if (!film) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null film");
+    if (w == 0 || h == 0) return set_error(ctx, SPX_EINVAL, std::string(who) + ": zero-size film");
+    if ((uint64_t)w * h > (1ull << 28))
+        return set_error(ctx, SPX_ECAP, std::string(who) + ": film larger than 2^28 pixels");
+    return SPX_OK;
+}
+
+int spx::photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_opts *opts,
+                       uint32_t n, uint32_t photon_base, uint32_t seed, PhotonParams &p, const char *who) {
+    if (!beam) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null beam");
+    if (!finite3(beam->corner) || !finite3(beam->u) || !finite3(beam->v) || !finite3(beam->dir))
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite beam");
+    spx_opts o = opts ? *opts : default_opts(false);
+    if (!valid_opts(&o, false)) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad options");
+    memset(&p, 0, sizeof p);
+    p.beam = *beam;
+    p.n = n;
+    p.photon_base = photon_base;
+    p.seed = seed;
+    p.illuminant = o.illuminant;
+    p.dispersion = o.dispersion;
+    p.max_depth = o.max_depth;
+    p.ill_scale = illuminant_scale(ctx, o.illuminant);
+    p.replicas = 1;
+    p.rep_stride = 0;
+    p.scene = scene->dev;
+    p.tbl = table_for(ctx, o.illuminant);
+    return SPX_OK;
+}
+
+void spx::set_screen(PhotonParams &p, const spx_screen &scr) {
+    // Screen::new, lighttrace.rs:27-34
+    f3 u = ld3(scr.u), v = ld3(scr.v), n = normalize(cross(u, v));
+    memcpy(p.scr_corner, scr.corner, sizeof p.scr_corner);
+    memcpy(p.scr_u, scr.u, sizeof p.scr_u);
+    memcpy(p.scr_v, scr.v, sizeof p.scr_v);
+    p.scr_n[0] = n.x; p.scr_n[1] = n.y; p.scr_n[2] = n.z;
+    p.scr_uu = dot(u, u);
+    p.scr_vv = dot(v, v);
+    p.scr_w = scr.width;
+    p.scr_h = scr.height;
+}
 
 extern "C" {
 
@@ -347,14 +426,21 @@ int spx_ctx_create(int device, spx_ctx **out) {
     ctx->tuning.splat_mode = 2;
     ctx->tuning.chunk_log2 = 22;
     ctx->tuning.bin_log2 = 30;
+    ctx->tuning.tile_wlog2 = kTileLog2;
+    ctx->tuning.batch_blocks = 0;
+    ctx->tuning.chunk_paths = 16384;
+    ctx->tuning.chunk_photons = 8192;
     *out = ctx;
     return SPX_OK;
 }
 
 int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
     if (!ctx || !t) return SPX_EINVAL;
-    if (t->max_replicas < 1 || t->max_replicas > 256 || t->splat_mode > 2 || t->chunk_log2 < 12 ||
-        t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30)
+    if (t->max_replicas < 1 || t->max_replicas > 256 || t->splat_mode > 3 || t->chunk_log2 < 12 ||
+        t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30 ||
+        t->tile_wlog2 < kTileLog2Min || t->tile_wlog2 > kTileLog2Max || t->batch_blocks > 65536 ||
+        t->chunk_paths < 64 || t->chunk_paths > (1u << 22) || t->chunk_photons < 64 ||
+        t->chunk_photons > (1u << 24))
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
     return SPX_OK;
@@ -396,22 +482,36 @@ int spx_ctx_kernel_timing(spx_ctx *ctx, int enable) {
     return SPX_OK;
 }
 
-int spx_ctx_kernel_time(spx_ctx *ctx, double *total_ms, uint32_t *launches) {
-    if (!ctx || !total_ms || !launches) return set_error(ctx, SPX_EINVAL, "spx_ctx_kernel_time: null argument");
-    double sum = 0.0;
+int spx_ctx_kernel_times(spx_ctx *ctx, double ms[3], uint32_t launches[3]) {
+    if (!ctx || !ms || !launches) return set_error(ctx, SPX_EINVAL, "spx_ctx_kernel_times: null argument");
+    for (int g = 0; g < 3; ++g) {
+        ms[g] = 0.0;
+        launches[g] = 0;
+    }
     for (size_t k = 0; k + 1 < ctx->ktime_used; k += 2) {
         hipError_t e = hipEventSynchronize(ctx->ktime_ev[k + 1]);
-        float ms = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ktime_ev[k], ctx->ktime_ev[k + 1]);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ktime_ev[k], ctx->ktime_ev[k + 1]);
         if (e != hipSuccess) {
             ctx->ktime_used = 0;
-            return check_hip(ctx, e, "spx_ctx_kernel_time");
+            return check_hip(ctx, e, "spx_ctx_kernel_times");
         }
-        sum += ms;
+        uint32_t g = ctx->ktime_kind[k / 2];
+        ms[g] += t;
+        launches[g] += 1;
     }
-    *total_ms = sum;
-    *launches = (uint32_t)(ctx->ktime_used / 2);
     ctx->ktime_used = 0;
+    return SPX_OK;
+}
+
+int spx_ctx_kernel_time(spx_ctx *ctx, double *total_ms, uint32_t *launches) {
+    if (!ctx || !total_ms || !launches) return set_error(ctx, SPX_EINVAL, "spx_ctx_kernel_time: null argument");
+    double ms[3];
+    uint32_t n[3];
+    int rc = spx_ctx_kernel_times(ctx, ms, n);
+    if (rc) return rc;
+    *total_ms = ms[SPX_KT_CAMERA] + ms[SPX_KT_PHOTON];
+    *launches = n[SPX_KT_CAMERA] + n[SPX_KT_PHOTON];
     return SPX_OK;
 }
 
@@ -501,13 +601,6 @@ static int check_common(spx_ctx *ctx, const spx_scene *scene, const char *who) {
     return SPX_OK;
 }
 
-static int check_film(spx_ctx *ctx, const float *film, uint32_t w, uint32_t h, const char *who) {
-    if (!film) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null film");
-    if (w == 0 || h == 0) return set_error(ctx, SPX_EINVAL, std::string(who) + ": zero-size film");
-    if ((uint64_t)w * h > (1ull << 28))
-        return set_error(ctx, SPX_ECAP, std::string(who) + ": film larger than 2^28 pixels");
-    return SPX_OK;
-}
 
 int spx_camera_accumulate(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, uint32_t width,
                           uint32_t height, uint32_t spp, uint32_t sample_base, uint32_t seed,
@@ -545,41 +638,7 @@ int spx_camera_accumulate(spx_ctx *ctx, const spx_scene *scene, const spx_camera
     return check_hip(ctx, e, "spx_camera_accumulate launch");
 }
 
-static int photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_opts *opts,
-                         uint32_t n, uint32_t photon_base, uint32_t seed, PhotonParams &p, const char *who) {
-    if (!beam) return set_error(ctx, SPX_EINVAL, std::string(who) + ": null beam");
-    if (!finite3(beam->corner) || !finite3(beam->u) || !finite3(beam->v) || !finite3(beam->dir))
-        return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite beam");
-    spx_opts o = opts ? *opts : default_opts(false);
-    if (!valid_opts(&o, false)) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad options");
-    memset(&p, 0, sizeof p);
-    p.beam = *beam;
-    p.n = n;
-    p.photon_base = photon_base;
-    p.seed = seed;
-    p.illuminant = o.illuminant;
-    p.dispersion = o.dispersion;
-    p.max_depth = o.max_depth;
-    p.ill_scale = illuminant_scale(ctx, o.illuminant);
-    p.replicas = 1;
-    p.rep_stride = 0;
-    p.scene = scene->dev;
-    p.tbl = table_for(ctx, o.illuminant);
-    return SPX_OK;
-}
 
-static void set_screen(PhotonParams &p, const spx_screen &scr) {
-    // Screen::new, lighttrace.rs:27-34
-    f3 u = ld3(scr.u), v = ld3(scr.v), n = normalize(cross(u, v));
-    memcpy(p.scr_corner, scr.corner, sizeof p.scr_corner);
-    memcpy(p.scr_u, scr.u, sizeof p.scr_u);
-    memcpy(p.scr_v, scr.v, sizeof p.scr_v);
-    p.scr_n[0] = n.x; p.scr_n[1] = n.y; p.scr_n[2] = n.z;
-    p.scr_uu = dot(u, u);
-    p.scr_vv = dot(v, v);
-    p.scr_w = scr.width;
-    p.scr_h = scr.height;
-}
 
 int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *screen, const spx_beam *beam,
                    uint32_t n, uint32_t photon_base, uint32_t seed, const spx_opts *opts, uint32_t engine,
@@ -600,10 +659,11 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        if (ctx->tuning.splat_mode == 2)
+        if (ctx->tuning.splat_mode >= 2)
             e = with_bins(ctx, p, screen->width, screen->height, std::min(n, chunk), 1, s,
                           [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
-                              return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s, b, sh);
+                              return run_photon_wavefront(ctx, q, 1, ctx->scratch, chunk, s, b, sh,
+                                                          ctx->tuning.splat_mode == 3);
                           }, &rc);
         else
             e = with_replicas(ctx, p, (size_t)3 * screen->width * screen->height, s,
@@ -646,10 +706,11 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;
-        if (ctx->tuning.splat_mode == 2)
+        if (ctx->tuning.splat_mode >= 2)
             e = with_bins(ctx, p, width, height, std::min(n, chunk), 0, s,
                           [&](const PhotonParams &q, const BinState *b, const uint32_t *sh) {
-                              return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s, b, sh);
+                              return run_photon_wavefront(ctx, q, 0, ctx->scratch, chunk, s, b, sh,
+                                                          ctx->tuning.splat_mode == 3);
                           }, &rc);
         else
             e = with_replicas(ctx, p, (size_t)3 * width * height, s,
@@ -731,317 +792,6 @@ int spx_glass_n(spx_ctx *ctx, uint32_t glass, const float *lambda_host, uint32_t
     return check_hip(ctx, e, "spx_glass_n");
 }
 
-// ---- batched jobs ------------------------------------------------------------
-static int build_batch_job(spx_ctx *ctx, const spx_job &jb, BatchJob &out, uint32_t idx) {
-    char who[64];
-    snprintf(who, sizeof who, "spx_batch_run job %u", idx);
-    memset(&out, 0, sizeof out);
-    out.kind = jb.kind;
-    if (!jb.scene || jb.scene->ctx != ctx) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad scene");
-    int rc;
-    if (jb.kind == SPX_JOB_CAMERA) {
-        if ((rc = check_film(ctx, jb.film, jb.width, jb.height, who))) return rc;
-        if (!valid_opts(&jb.opts, true)) return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad options");
-        if (!finite3(jb.camera.origin) || !finite3(jb.camera.lower_left) || !finite3(jb.camera.horizontal) ||
-            !finite3(jb.camera.vertical))
-            return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite camera");
-        CamParams &p = out.cam;
-        p.cam = jb.camera;
-        p.w = jb.width; p.h = jb.height; p.spp = jb.spp; p.sample_base = jb.sample_base; p.seed = jb.seed;
-        p.illuminant = jb.opts.illuminant; p.dispersion = jb.opts.dispersion;
-        p.max_depth = jb.opts.max_depth; p.rr_depth = jb.opts.rr_depth;
-        p.scene = jb.scene->dev;
-        p.tbl = table_for(ctx, jb.opts.illuminant);
-        p.film = jb.film;
-        return SPX_OK;
-    }
-    if (jb.kind != SPX_JOB_VOLUME && jb.kind != SPX_JOB_LIGHTTRACE)
-        return set_error(ctx, SPX_EINVAL, std::string(who) + ": unknown kind");
-    PhotonParams &p = out.ph;
-    if ((rc = photon_params(ctx, jb.scene, &jb.beam, &jb.opts, jb.n_photons, jb.photon_base, jb.seed, p, who)))
-        return rc;
-    if (jb.kind == SPX_JOB_LIGHTTRACE) {
-        if ((rc = check_film(ctx, jb.film, jb.screen.width, jb.screen.height, who))) return rc;
-        set_screen(p, jb.screen);
-    } else {
-        if ((rc = check_film(ctx, jb.film, jb.width, jb.height, who))) return rc;
-        if (jb.estimator != SPX_EST_POINT && jb.estimator != SPX_EST_LEGACY)
-            return set_error(ctx, SPX_EINVAL, std::string(who) + ": batches run the point or legacy estimator");
-        const spx_vol_params &vp = jb.vol;
-        if (!isfinite(vp.sigma_s) || !isfinite(vp.sigma_t) || !isfinite(vp.g) || !(vp.max_dist > 0.0f) ||
-            !isfinite(vp.max_dist))
-            return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad volume parameters");
-        if (!finite3(jb.camera.origin) || !finite3(jb.camera.lower_left) || !finite3(jb.camera.horizontal) ||
-            !finite3(jb.camera.vertical))
-            return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite camera");
-        make_proj(jb.camera, p.proj);
-        p.vp = vp;
-        p.w = jb.width;
-        p.h = jb.height;
-        p.estimator = jb.estimator;
-        p.zbuf = jb.zbuf;
-    }
-    p.film = jb.film;
-    return SPX_OK;
-}
-
-static int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
-    if (ctx->batch_pending) {  // the previous staging copy may still read the pinned buffer
-        hipError_t e = hipEventSynchronize(ctx->batch_done);
-        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run wait");
-        ctx->batch_pending = false;
-    }
-    if (!ctx->batch_done) {
-        hipError_t e = hipEventCreateWithFlags(&ctx->batch_done, hipEventDisableTiming);
-        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run event");
-    }
-    if (ctx->batch_host_bytes < bytes) {
-        if (ctx->batch_host) hipHostFree(ctx->batch_host);
-        ctx->batch_host = nullptr;
-        ctx->batch_host_bytes = 0;
-        hipError_t e = hipHostMalloc(&ctx->batch_host, bytes * 2, hipHostMallocDefault);
-        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run pinned staging");
-        ctx->batch_host_bytes = bytes * 2;
-    }
-    if (ctx->batch_dev_bytes < bytes) {
-        hipError_t e = hipStreamSynchronize(s);  // earlier launches may still read the old table
-        if (e == hipSuccess) e = hipDeviceSynchronize();
-        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run sync");
-        if (ctx->batch_dev) hipFree(ctx->batch_dev);
-        ctx->batch_dev = nullptr;
-        ctx->batch_dev_bytes = 0;
-        e = hipMalloc(&ctx->batch_dev, bytes * 2);
-        if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run table");
-        ctx->batch_dev_bytes = bytes * 2;
-    }
-    return SPX_OK;
-}
-
-int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_batch_cfg *cfg, void *stream) {
-    if (!ctx) return SPX_EINVAL;
-    if (!jobs && n_jobs) return set_error(ctx, SPX_EINVAL, "spx_batch_run: null job table");
-    if (n_jobs == 0) return SPX_OK;
-    hipStream_t s = (hipStream_t)stream;
-    uint32_t chunk_paths = cfg && cfg->chunk_paths ? cfg->chunk_paths : 16384u;
-    uint32_t chunk_photons = cfg && cfg->chunk_photons ? cfg->chunk_photons : 8192u;
-    uint32_t blocks = cfg && cfg->blocks ? cfg->blocks : (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
-    uint32_t splat_mode = cfg ? cfg->splat_mode : ctx->tuning.splat_mode;
-    if (splat_mode > 2) return set_error(ctx, SPX_EINVAL, "spx_batch_run: unknown splat mode");
-    const bool binned = splat_mode == 2;
-    std::vector<BatchJob> table(n_jobs);
-    std::vector<BatchItem> items;
-    std::vector<double> job_cost(n_jobs, 0.0);
-    std::vector<std::pair<const float *, uint32_t>> films;
-    for (uint32_t j = 0; j < n_jobs; ++j) {
-        int rc = build_batch_job(ctx, jobs[j], table[j], j);
-        if (rc) return rc;
-        films.push_back({jobs[j].film, jobs[j].kind});
-        const spx_job &jb = jobs[j];
-        if (jb.kind == SPX_JOB_CAMERA) {
-            if (jb.spp == 0) continue;
-            uint32_t npix = jb.width * jb.height;
-            uint32_t per = chunk_paths / jb.spp;
-            per = per < 1 ? 1 : (per > 2048 ? 2048 : per);
-            for (uint32_t p0 = 0; p0 < npix; p0 += per) {
-                uint32_t c = npix - p0 < per ? npix - p0 : per;
-                uint64_t cost = (uint64_t)c * jb.spp;
-                items.push_back({j, p0, c, (uint32_t)std::min<uint64_t>(cost, 0xFFFFFFFFu)});
-                job_cost[j] += (double)cost;
-            }
-        } else {
-            for (uint32_t p0 = 0; p0 < jb.n_photons; p0 += chunk_photons) {
-                uint32_t c = jb.n_photons - p0 < chunk_photons ? jb.n_photons - p0 : chunk_photons;
-                items.push_back({j, p0, c, 2u * c});
-                job_cost[j] += 2.0 * c;
-            }
-        }
-    }
-    // Camera films are written without atomics: no two jobs may share one.
-    std::vector<std::pair<const float *, uint32_t>> sorted_films = films;
-    std::sort(sorted_films.begin(), sorted_films.end());
-    for (size_t k = 1; k < sorted_films.size(); ++k)
-        if (sorted_films[k].first == sorted_films[k - 1].first &&
-            (sorted_films[k].second == SPX_JOB_CAMERA || sorted_films[k - 1].second == SPX_JOB_CAMERA))
-            return set_error(ctx, SPX_EINVAL, "spx_batch_run: a camera job shares its film with another job");
-    if (items.empty()) return SPX_OK;
-
-    // Binned deposits: key ranges per photon job; photon items split into
-    // rounds of at most kBinFill of the record capacity (estimated deposits
-    // per photon from records_per_photon; overflow falls back to atomics, so
-    // the estimate only affects speed).
-    // Round 0 holds the camera items (camera kernel); photon items follow in
-    // rounds 1.. (photon kernel).
-    BinJobMap map;
-    size_t bin_want = 0;
-    std::vector<uint32_t> item_round(items.size(), 1);
-    for (size_t k = 0; k < items.size(); ++k)
-        if (jobs[items[k].job].kind == SPX_JOB_CAMERA) item_round[k] = 0;
-    uint32_t n_rounds = 2;
-    if (binned) {
-        std::map<const float *, uint32_t> first_owner;
-        for (uint32_t j = 0; j < n_jobs; ++j) {
-            if (jobs[j].kind == SPX_JOB_CAMERA) continue;
-            const bool lt = jobs[j].kind == SPX_JOB_LIGHTTRACE;
-            uint32_t fw = lt ? jobs[j].screen.width : jobs[j].width, fh = lt ? jobs[j].screen.height : jobs[j].height;
-            if (((uint64_t)map.n_strips + film_strips(fw, fh)) * kStripPix > 0xFFFFFFFFull)
-                return set_error(ctx, SPX_ECAP, "spx_batch_run: photon films exceed 2^32 tile pixels");
-            table[j].ph.key_base = map.add(jobs[j].film, fw, fh);
-            table[j].ph.replicas = 1;
-            table[j].ph.rep_stride = 0;
-            auto it = first_owner.find(jobs[j].film);
-            if (it != first_owner.end()) {
-                map.shared.back() = 1;
-                map.shared[it->second] = 1;
-            } else {
-                first_owner[jobs[j].film] = (uint32_t)map.film.size() - 1;
-            }
-        }
-        const double budget = kBinFill * (double)((size_t)1 << ctx->tuning.bin_log2);
-        double used = 0.0, most = 0.0;
-        uint32_t r = 1;
-        for (size_t k = 0; k < items.size(); ++k) {
-            const spx_job &jb = jobs[items[k].job];
-            if (jb.kind == SPX_JOB_CAMERA) continue;
-            double est = (double)items[k].count * records_per_photon(jb.kind == SPX_JOB_LIGHTTRACE);
-            if (used > 0.0 && used + est > budget) {
-                ++r;
-                used = 0.0;
-            }
-            used += est;
-            most = std::max(most, used);
-            item_round[k] = r;
-        }
-        n_rounds = r + 1;
-        bin_want = (size_t)(most / kBinFill);
-    }
-
-    // Film replicas for the photon jobs (atomic deposit modes only).
-    std::vector<ReplicaJob> reps;
-    size_t rep_total = 0;
-    uint32_t rep_blocks = 0;
-    for (uint32_t j = 0; j < n_jobs && !binned; ++j) {
-        if (jobs[j].kind == SPX_JOB_CAMERA || jobs[j].n_photons == 0) continue;
-        size_t n = jobs[j].kind == SPX_JOB_LIGHTTRACE ? (size_t)3 * jobs[j].screen.width * jobs[j].screen.height
-                                                      : (size_t)3 * jobs[j].width * jobs[j].height;
-        uint32_t R = replica_count(ctx, n);
-        if (R <= 1) continue;
-        ReplicaJob rj;
-        rj.dst = jobs[j].film;
-        rj.src = (const float *)(uintptr_t)rep_total;  // offset, rebased below
-        rj.n = (uint32_t)n;
-        rj.reps = R;
-        rj.first_block = rep_blocks;
-        rj.pad = j;
-        reps.push_back(rj);
-        rep_total += (size_t)R * n;
-        rep_blocks += (uint32_t)((n + 1023) / 1024);
-    }
-    if (!reps.empty()) {
-        int rc2 = ensure_replicas(ctx, rep_total, s);
-        if (rc2) return rc2;
-        for (ReplicaJob &rj : reps) {
-            size_t off = (size_t)(uintptr_t)rj.src;
-            rj.src = ctx->replica + off;
-            PhotonParams &p = table[rj.pad].ph;
-            p.film = ctx->replica + off;
-            p.replicas = rj.reps;
-            p.rep_stride = rj.n;
-        }
-        hipError_t ef = launch_fill(ctx->replica, 0.0f, rep_total, s);
-        if (ef != hipSuccess) return check_hip(ctx, ef, "spx_batch_run replica clear");
-    }
-
-    // Jobs to XCD queues, largest first onto the least-loaded queue.
-    std::vector<uint32_t> order(n_jobs);
-    for (uint32_t j = 0; j < n_jobs; ++j) order[j] = j;
-    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return job_cost[a] > job_cost[b]; });
-    std::vector<uint32_t> job_queue(n_jobs, 0);
-    double load[kXcdQueues] = {0};
-    for (uint32_t j : order) {
-        uint32_t best = 0;
-        for (uint32_t q = 1; q < kXcdQueues; ++q)
-            if (load[q] < load[best]) best = q;
-        job_queue[j] = best;
-        load[best] += job_cost[j];
-    }
-    std::vector<uint32_t> idx(items.size());
-    for (size_t k = 0; k < items.size(); ++k) idx[k] = (uint32_t)k;
-    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) {
-        if (item_round[a] != item_round[b]) return item_round[a] < item_round[b];
-        uint32_t qa = job_queue[items[a].job], qb = job_queue[items[b].job];
-        if (qa != qb) return qa < qb;
-        return items[a].cost > items[b].cost;
-    });
-    std::vector<BatchItem> ordered(items.size());
-    std::vector<BatchQueues> qh(n_rounds);
-    memset(qh.data(), 0, n_rounds * sizeof(BatchQueues));
-    for (size_t k = 0; k < idx.size(); ++k) {
-        ordered[k] = items[idx[k]];
-        qh[item_round[idx[k]]].size[job_queue[items[idx[k]].job]]++;
-    }
-    uint32_t start = 0;
-    for (uint32_t r = 0; r < n_rounds; ++r)
-        for (uint32_t q = 0; q < kXcdQueues; ++q) {
-            qh[r].begin[q] = start;
-            start += qh[r].size[q];
-        }
-    const size_t qstride = (sizeof(BatchQueues) + 255) & ~(size_t)255;
-    size_t off_items = qstride * n_rounds;
-    size_t off_jobs = (off_items + ordered.size() * sizeof(BatchItem) + 255) & ~(size_t)255;
-    size_t off_reps = (off_jobs + table.size() * sizeof(BatchJob) + 255) & ~(size_t)255;
-    size_t bytes = off_reps + reps.size() * sizeof(ReplicaJob);
-    BinState bst;
-    memset(&bst, 0, sizeof bst);
-    const uint32_t *job_shared = nullptr;
-    if (binned) {
-        int rc2 = prepare_bins(ctx, map, bin_want, s, bst, &job_shared);
-        if (rc2) return rc2;
-    }
-    int rc = batch_stage(ctx, bytes, s);
-    if (rc) return rc;
-    char *h = (char *)ctx->batch_host;
-    memset(h, 0, bytes);
-    for (uint32_t r = 0; r < n_rounds; ++r) memcpy(h + r * qstride, &qh[r], sizeof(BatchQueues));
-    memcpy(h + off_items, ordered.data(), ordered.size() * sizeof(BatchItem));
-    memcpy(h + off_jobs, table.data(), table.size() * sizeof(BatchJob));
-    if (!reps.empty()) memcpy(h + off_reps, reps.data(), reps.size() * sizeof(ReplicaJob));
-    char *d = (char *)ctx->batch_dev;
-    hipError_t e = hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
-    if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run upload");
-    ctx->batch_pending = true;
-    const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
-    for (uint32_t r = 0; r < n_rounds && e == hipSuccess; ++r) {
-        uint32_t round_items = 0;
-        for (uint32_t q = 0; q < kXcdQueues; ++q) round_items += qh[r].size[q];
-        if (round_items == 0) continue;
-        const bool camera = r == 0;
-        if (binned && !camera) e = launch_bins_reset(bst, s);
-        hipEvent_t ev_end = nullptr;
-        if (e == hipSuccess && ctx->ktime_on) {
-            while (ctx->ktime_ev.size() < ctx->ktime_used + 2) {
-                hipEvent_t ev;
-                e = hipEventCreate(&ev);
-                if (e != hipSuccess) break;
-                ctx->ktime_ev.push_back(ev);
-            }
-            if (e == hipSuccess) e = hipEventRecord(ctx->ktime_ev[ctx->ktime_used], s);
-            if (e == hipSuccess) ev_end = ctx->ktime_ev[ctx->ktime_used + 1];
-        }
-        if (e == hipSuccess)
-            e = launch_batch(camera, (const BatchJob *)(d + off_jobs), (const BatchItem *)(d + off_items),
-                             (BatchQueues *)(d + r * qstride), blocks, splat_mode, bst, s);
-        if (e == hipSuccess && ev_end) {
-            e = hipEventRecord(ev_end, s);
-            if (e == hipSuccess) ctx->ktime_used += 2;
-        }
-        if (e == hipSuccess && binned && !camera) e = launch_bins_resolve(bst, job_shared, acc_blocks, s);
-    }
-    if (e == hipSuccess && !reps.empty())
-        e = launch_reduce_replicas((const ReplicaJob *)(d + off_reps), (uint32_t)reps.size(), rep_blocks, s);
-    return check_hip(ctx, e, "spx_batch_run launch");
-}
-
 // ---- memory / streams / events ----------------------------------------------
 int spx_malloc(spx_ctx *ctx, size_t bytes, void **out) {
     if (!ctx || !out || bytes == 0) return set_error(ctx, SPX_EINVAL, "spx_malloc: bad arguments");
@@ -1116,10 +866,3 @@ int spx_event_elapsed_ms(spx_ctx *ctx, void *a, void *b, float *ms) {
 }
 
 }  // extern "C"
-
-namespace spx {
-int validate_job(spx_ctx *ctx, const spx_job &job) {
-    BatchJob tmp;
-    return build_batch_job(ctx, job, tmp, 0);
-}
-}  // namespace spx

@@ -6,8 +6,9 @@
 // one 512-record chunk at a time, so an append costs one LDS read-modify-
 // write by the wave's leader and one coalesced 16-byte store per lane; a new
 // chunk costs one global atomic per 512 records.  After the trace, the
-// records are counted per strip (a 32x32-pixel film tile: spatially
-// compact, so the few tiles a beam or caustic crosses collect its deposits),
+// records are counted per strip (a 1024-pixel film tile, 2^tw x 2^(10-tw)
+// with tw chosen per call -- 32x32 by default: spatially compact, so the
+// few tiles a beam or caustic crosses collect its deposits),
 // scanned, scattered into strip order and accumulated strip by strip in LDS
 // as 64-bit fixed point scaled to the strip's largest deposit and record
 // count (integer LDS atomics: order-independent, resolution finer than the
@@ -25,8 +26,10 @@ namespace spx {
 
 constexpr uint32_t kBinChunk = 512;      // records per wave chunk
 constexpr uint32_t kChunksPerBlock = 8;  // chunks per count / scatter block
-constexpr uint32_t kStripLog2 = 10;      // 1024 film pixels per strip (a 32x32 tile)
-constexpr uint32_t kTileLog2 = 5;        // tile side 32
+constexpr uint32_t kStripLog2 = 10;      // 1024 film pixels per strip (one film tile)
+constexpr uint32_t kTileLog2 = 5;        // default tile width log2 (32x32 tiles)
+constexpr uint32_t kTileLog2Min = 2;     // tile shapes from 4x256 ...
+constexpr uint32_t kTileLog2Max = 8;     // ... to 256x4
 constexpr uint32_t kStripPix = 1u << kStripLog2;
 constexpr uint32_t kBinPools = 8;        // one per XCD
 constexpr uint32_t kPartRecs = 16384;    // records per accumulation work item
@@ -82,6 +85,7 @@ struct BinState {
     float *const *job_film;      // [n_jobs] film pointers
     uint32_t pool_cap;     // records per pool (multiple of kBinChunk)
     uint32_t n_strips;
+    uint32_t tile_wlog2;   // tiles are 2^tile_wlog2 wide and 2^(10 - tile_wlog2) tall
 };
 
 __device__ inline uint32_t xcc_id() {
@@ -89,13 +93,27 @@ __device__ inline uint32_t xcc_id() {
     return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & 0xFu;
 }
 
-// Key of film pixel idx within its job: tile index * 1024 + pixel in tile.
-__host__ __device__ inline uint32_t tile_key(uint32_t idx, uint32_t width) {
+// Tiles across a film of `width` pixels for tiles 2^tw wide.
+__host__ __device__ inline uint32_t tiles_across(uint32_t width, uint32_t tw) {
+    return (width + (1u << tw) - 1) >> tw;
+}
+
+// Key of film pixel idx within its job: tile index * 1024 + pixel in tile,
+// tiles 2^tw wide and 2^(10 - tw) tall, row-major in the film and inside.
+__host__ __device__ inline uint32_t tile_key(uint32_t idx, uint32_t width, uint32_t tiles_x, uint32_t tw) {
+    const uint32_t th = kStripLog2 - tw;
     uint32_t px = idx % width, py = idx / width;
-    uint32_t tiles_x = (width + (1u << kTileLog2) - 1) >> kTileLog2;
-    uint32_t tile = (py >> kTileLog2) * tiles_x + (px >> kTileLog2);
-    uint32_t local = ((py & ((1u << kTileLog2) - 1)) << kTileLog2) | (px & ((1u << kTileLog2) - 1));
+    uint32_t tile = (py >> th) * tiles_x + (px >> tw);
+    uint32_t local = ((py & ((1u << th) - 1)) << tw) | (px & ((1u << tw) - 1));
     return (tile << kStripLog2) | local;
+}
+
+// Film pixel (px, py) of in-tile position k of tile `tile` (tile_key's inverse).
+__host__ __device__ inline void tile_pixel(uint32_t tile, uint32_t k, uint32_t tiles_x, uint32_t tw, uint32_t &px,
+                                           uint32_t &py) {
+    const uint32_t th = kStripLog2 - tw;
+    px = ((tile % tiles_x) << tw) + (k & ((1u << tw) - 1));
+    py = ((tile / tiles_x) << th) + (k >> tw);
 }
 
 // A chunk of `pool` (this XCD's), else of another pool with room, else
@@ -126,7 +144,9 @@ struct BinSink {
     uint32_t key_base;
     float *film;         // fallback when the pool is exhausted
     uint32_t *wstate;
-    uint32_t width;      // film width in pixels (the tile grid is ceil(width / 32) wide)
+    uint32_t width;      // film width in pixels
+    uint32_t tiles_x;    // tiles across the film
+    uint32_t tile_wlog2; // tile width log2
 
     __device__ inline void add(uint32_t idx, f3 v) {
         uint64_t m = __ballot(1);
@@ -158,7 +178,7 @@ struct BinSink {
             return;
         }
         BinRec r;
-        r.key = key_base + tile_key(idx, width);
+        r.key = key_base + tile_key(idx, width, tiles_x, tile_wlog2);
         r.x = v.x;
         r.y = v.y;
         r.z = v.z;

@@ -73,8 +73,10 @@ struct spx_ctx {
     size_t batch_host_bytes = 0;
     hipEvent_t batch_done = nullptr;  // recorded after the staging copy
     bool batch_pending = false;
-    bool ktime_on = false;             // bracket k_batch launches with events
+    uint64_t buf_gen = 0;              // bumped whenever bins / replica / batch_dev move
+    bool ktime_on = false;             // bracket batched kernel groups with events
     std::vector<hipEvent_t> ktime_ev;  // pool; pairs (start, end)
+    std::vector<uint32_t> ktime_kind;  // SPX_KT_* of each recorded pair
     size_t ktime_used = 0;             // events recorded since the last read
 };
 
@@ -130,6 +132,8 @@ struct BinJobMap {
     std::vector<uint32_t> key, npix, width, shared, strip_job;
     std::vector<float *> film;
     uint32_t n_strips = 0;
+    uint32_t tile_wlog2 = kTileLog2;  // tiles 2^tile_wlog2 x 2^(10 - tile_wlog2)
+    explicit BinJobMap(uint32_t tw = kTileLog2) : tile_wlog2(tw) {}
     // Adds a film of w x h pixels; returns its key base.
     uint32_t add(float *f, uint32_t w, uint32_t h) {
         uint32_t k = n_strips << kStripLog2;
@@ -138,16 +142,34 @@ struct BinJobMap {
         width.push_back(w);
         shared.push_back(0);
         film.push_back(f);
-        uint32_t t = 1u << kTileLog2;
-        uint32_t strips = ((w + t - 1) >> kTileLog2) * ((h + t - 1) >> kTileLog2);
+        uint32_t strips = film_strips(w, h, tile_wlog2);
         for (uint32_t i = 0; i < strips; ++i) strip_job.push_back((uint32_t)film.size() - 1);
         n_strips += strips;
         return k;
     }
+    static uint32_t film_strips(uint32_t w, uint32_t h, uint32_t tw) {
+        const uint32_t th = kStripLog2 - tw;
+        return ((w + (1u << tw) - 1) >> tw) * ((h + (1u << th) - 1) >> th);
+    }
 };
-// Allocates the round buffers for `map` with room for `want` records (a
-// power of two within [2^15, 2^bin_log2]), uploads the map on `s` and fills
-// `st` / `job_shared`.
+// Byte layout of one deposit round's buffers: records, sorted records,
+// counters, group lists, split-strip accumulators, then the job map.
+struct BinLayout {
+    size_t cap = 0, n = 0, nj = 0, zero_words = 0, max_multi = 0;
+    size_t o_recs = 0, o_sorted = 0, o_pool = 0, o_partn = 0, o_fill = 0, o_zero = 0, o_tot = 0, o_max = 0;
+    size_t o_off = 0, o_poff = 0, o_glist = 0, o_gcount = 0, o_macc = 0, o_smacc = 0, o_maccs = 0, o_maccn = 0;
+    size_t o_map = 0, m_key = 0, m_npix = 0, m_w = 0, m_sh = 0, m_film = 0, m_end = 0;
+    size_t total = 0;
+};
+size_t a256(size_t x);
+// Layout for `map` with room for `want` records (capacity: a power of two
+// within [2^15, 2^bin_log2]).
+BinLayout bins_layout(const BinJobMap &map, size_t want, uint32_t bin_log2);
+// Points `st` / `job_shared` into the buffer at `base`.
+void bins_bind(char *base, const BinLayout &L, const BinJobMap &map, BinState &st, const uint32_t **job_shared);
+// Stream-ordered upload of the job map into the buffer at `base`.
+int bins_upload_map(spx_ctx *ctx, char *base, const BinLayout &L, const BinJobMap &map, hipStream_t s);
+// The above on the context's grow-only buffer.
 int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
                  const uint32_t **job_shared);
 // Share of the record capacity a round is planned to fill.
@@ -156,17 +178,31 @@ constexpr double kBinFill = 0.85;
 // average 13.9 (point estimator), screen photons under 4.
 inline double records_per_photon(bool lighttrace) { return lighttrace ? 4.0 : 14.0; }
 hipError_t launch_bins_reset(const BinState &st, hipStream_t s);
-// Tile grid size of a film, in strips (32x32 tiles).
-inline uint32_t film_strips(uint32_t w, uint32_t h) {
-    uint32_t t = 1u << kTileLog2;
-    return ((w + t - 1) >> kTileLog2) * ((h + t - 1) >> kTileLog2);
-}
 hipError_t launch_bins_resolve(const BinState &st, const uint32_t *job_shared, uint32_t acc_blocks, hipStream_t s);
 // Stream-ordered upload of a small host block through pinned staging.
 int upload_async(spx_ctx *ctx, void *dev, const void *host, size_t bytes, hipStream_t s);
 
 // Validates a batch job exactly as spx_batch_run would (no launch).
 int validate_job(spx_ctx *ctx, const spx_job &job);
+// A batch plan on the context's own grow-only buffers (no allocation when
+// they are large enough): for callers that update the plan before every
+// launch (the server).  Launching it after the context reallocated those
+// buffers, or after another batch ran on the context without an update in
+// between, is invalid; spx_batch_plan_update refreshes the uploads and
+// fails with SPX_EINVAL once the buffers have moved.
+int batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, hipStream_t s,
+                             spx_batch_plan **out);
+// Records per frame a photon job is expected to deposit (its queue share).
+double job_queue_bytes(const spx_ctx *ctx, const spx_job &job);
+
+// Argument checks and parameter blocks shared by the entry points.
+bool valid_opts(const spx_opts *o, bool camera);
+spx_opts default_opts(bool camera);
+bool finite3(const float *v);
+int check_film(spx_ctx *ctx, const float *film, uint32_t w, uint32_t h, const char *who);
+int photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_opts *opts, uint32_t n,
+                  uint32_t photon_base, uint32_t seed, PhotonParams &p, const char *who);
+void set_screen(PhotonParams &p, const spx_screen &scr);
 int atomic_probe(spx_ctx *ctx, float *buf, size_t n_slots, uint32_t ops, uint32_t threads, hipStream_t s);
 
 // Wavefront engine (per-bounce SoA queues).
@@ -174,8 +210,10 @@ size_t camera_wavefront_scratch(uint32_t chunk);
 hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch, uint32_t chunk,
                                 hipStream_t s);
 size_t photon_wavefront_scratch(uint32_t chunk);
+// bins: binned deposits (hash: an LDS hash in front of the records), else
+// the LDS hash draining to film atomics.
 hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,
                                 uint32_t chunk, hipStream_t s, const BinState *bins,
-                                const uint32_t *job_shared);
+                                const uint32_t *job_shared, bool hash = false);
 
 }  // namespace spx

@@ -1,10 +1,13 @@
 // spx_serve.hip -- the serving loop: continuous batching of render requests.
 //
-// Requests are admitted, oldest first, when their footprint (film + deposit
-// replicas) fits under the memory cap; every active request's next frame
-// runs in ONE batched launch per step (spx_batch_run), so requests that
-// arrive while others are on the GPU join at the next step and their work
-// overlaps on the device.  Films and RNG counters (frames_done) stay
+// Requests are admitted, oldest first, when their footprint (film plus its
+// share of the deposit queues: records or film replicas) fits under the
+// memory cap; every active request's next frame
+// runs in ONE batched launch per step, so requests that arrive while others
+// are on the GPU join at the next step and their work overlaps on the
+// device.  The step's schedule is captured as a batch plan (a hipGraph) and
+// replayed, with the frames' RNG bases and cameras rewritten, for as long
+// as the set of active requests stays the same; a new set is captured anew.  Films and RNG counters (frames_done) stay
 // resident on the device between frames.  The reference has no server; its
 // nearest analogue is the viewer's progressive loop (spectral-viewer
 // main.rs:395-419), whose camera moves reset the accumulation
@@ -13,6 +16,7 @@
 #include <string.h>
 
 #include <chrono>
+#include <algorithm>
 #include <deque>
 #include <map>
 #include <string>
@@ -28,11 +32,20 @@ struct Req {
     spx_request r;
     uint32_t state = SPX_REQ_PENDING;
     uint32_t frames_done = 0;
-    uint64_t footprint = 0;
+    uint64_t footprint = 0;   // film + queue share
+    uint64_t film_bytes = 0;
+    uint64_t queue_bytes = 0; // deposit records (or film replicas) one frame needs
     double t_submit = 0, t_admit = 0, t_done = 0;
     float *film = nullptr;
     size_t film_floats = 0;
 };
+
+// Deposit queues are shared by the requests of a step and never exceed one
+// round's record capacity (binned deposits: records + sorted copy, 32 B
+// each), so the requests' queue shares are charged up to that bound.
+uint64_t queue_cap(const spx_ctx *ctx) {
+    return ctx->tuning.splat_mode >= 2 ? (uint64_t)32 << ctx->tuning.bin_log2 : UINT64_MAX;
+}
 
 size_t job_film_floats(const spx_job &j) {
     return j.kind == SPX_JOB_LIGHTTRACE ? (size_t)3 * j.screen.width * j.screen.height
@@ -69,11 +82,17 @@ spx_camera orbit_camera(const spx_camera &cam, const float c[3], double deg) {
 struct spx_server {
     spx_ctx *ctx = nullptr;
     uint64_t cap = 0;
-    uint64_t used = 0;
+    uint64_t film_used = 0;   // films of admitted requests
+    uint64_t queue_used = 0;  // their queue shares (charged up to queue_cap)
+    uint64_t used(const spx_ctx *c) const { return film_used + std::min(queue_used, queue_cap(c)); }
     uint64_t next_id = 1;
     std::map<uint64_t, Req> reqs;
     std::deque<uint64_t> pending;
     std::vector<uint64_t> active;
+    bool graphs = true;                 // steps replay captured plans
+    spx_batch_plan *plan = nullptr;     // plan of the active set `plan_ids`
+    std::vector<uint64_t> plan_ids;
+    uint64_t replays = 0, captures = 0; // steps served by a replay / by a new capture
     std::chrono::steady_clock::time_point t0;
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -100,6 +119,7 @@ int spx_server_create(spx_ctx *ctx, uint64_t mem_cap_bytes, spx_server **out) {
 void spx_server_destroy(spx_server *srv) {
     if (!srv) return;
     hipDeviceSynchronize();
+    if (srv->plan) spx_batch_plan_destroy(srv->plan);
     for (auto &kv : srv->reqs)
         if (kv.second.film) hipFree(kv.second.film);
     delete srv;
@@ -107,7 +127,20 @@ void spx_server_destroy(spx_server *srv) {
 
 double spx_server_clock(const spx_server *srv) { return srv ? srv->now() : 0.0; }
 
-uint64_t spx_server_bytes_in_use(const spx_server *srv) { return srv ? srv->used : 0; }
+int spx_server_set_graphs(spx_server *srv, int enable) {
+    if (!srv) return SPX_EINVAL;
+    srv->graphs = enable != 0;
+    return SPX_OK;
+}
+
+int spx_server_stats(const spx_server *srv, uint64_t *captures, uint64_t *replays) {
+    if (!srv) return SPX_EINVAL;
+    if (captures) *captures = srv->captures;
+    if (replays) *replays = srv->replays;
+    return SPX_OK;
+}
+
+uint64_t spx_server_bytes_in_use(const spx_server *srv) { return srv ? srv->used(srv->ctx) : 0; }
 
 int spx_server_submit(spx_server *srv, const spx_request *req, uint64_t *id) {
     if (!srv || !req || !id) return SPX_EINVAL;
@@ -124,10 +157,11 @@ int spx_server_submit(spx_server *srv, const spx_request *req, uint64_t *id) {
     Req q;
     q.r = *req;
     q.film_floats = job_film_floats(req->job);
-    uint64_t film_bytes = q.film_floats * sizeof(float);
-    uint64_t reps = req->job.kind == SPX_JOB_CAMERA ? 0 : replica_count(ctx, q.film_floats);
-    q.footprint = film_bytes * (1 + (reps > 1 ? reps : 0));
-    if (q.footprint > srv->cap) return set_error(ctx, SPX_EBUSY, "spx_server_submit: footprint exceeds the cap");
+    q.film_bytes = q.film_floats * sizeof(float);
+    q.queue_bytes = (uint64_t)job_queue_bytes(ctx, req->job);
+    q.footprint = q.film_bytes + q.queue_bytes;
+    if (q.film_bytes + std::min<uint64_t>(q.queue_bytes, queue_cap(ctx)) > srv->cap)
+        return set_error(ctx, SPX_EBUSY, "spx_server_submit: footprint exceeds the cap");
     q.t_submit = srv->now();
     *id = srv->next_id++;
     srv->reqs[*id] = q;
@@ -144,7 +178,7 @@ int spx_server_step(spx_server *srv, void *stream, uint32_t *n_active) {
     // smaller later ones may still be admitted.
     for (auto it = srv->pending.begin(); it != srv->pending.end();) {
         Req &q = srv->reqs[*it];
-        if (srv->used + q.footprint > srv->cap) {
+        if (srv->film_used + q.film_bytes + std::min(srv->queue_used + q.queue_bytes, queue_cap(ctx)) > srv->cap) {
             ++it;
             continue;
         }
@@ -158,7 +192,8 @@ int spx_server_step(spx_server *srv, void *stream, uint32_t *n_active) {
         if (e != hipSuccess) return check_hip(ctx, e, "spx_server_step clear");
         q.state = SPX_REQ_ACTIVE;
         q.t_admit = srv->now();
-        srv->used += q.footprint;
+        srv->film_used += q.film_bytes;
+        srv->queue_used += q.queue_bytes;
         srv->active.push_back(*it);
         it = srv->pending.erase(it);
     }
@@ -179,7 +214,35 @@ int spx_server_step(spx_server *srv, void *stream, uint32_t *n_active) {
         }
         jobs.push_back(j);
     }
-    int rc = spx_batch_run(ctx, jobs.data(), (uint32_t)jobs.size(), nullptr, stream);
+    int rc = SPX_OK;
+    if (!srv->graphs) {
+        rc = spx_batch_run(ctx, jobs.data(), (uint32_t)jobs.size(), nullptr, stream);
+    } else {
+        bool replayed = false;
+        if (srv->plan && srv->plan_ids == srv->active) {
+            rc = spx_batch_plan_update(srv->plan, jobs.data(), (uint32_t)jobs.size(), stream);
+            if (rc == SPX_OK) {
+                rc = spx_batch_plan_launch(srv->plan, stream);
+                if (rc) return rc;
+                replayed = true;
+                srv->replays++;
+            } else if (rc != SPX_EINVAL) {
+                return rc;
+            }
+        }
+        if (!replayed) {
+            if (srv->plan) {
+                spx_batch_plan_destroy(srv->plan);
+                srv->plan = nullptr;
+                srv->plan_ids.clear();
+            }
+            rc = batch_plan_create_shared(ctx, jobs.data(), (uint32_t)jobs.size(), s, &srv->plan);
+            if (rc) return rc;
+            srv->plan_ids = srv->active;
+            srv->captures++;
+            rc = spx_batch_plan_launch(srv->plan, stream);
+        }
+    }
     if (rc) return rc;
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return check_hip(ctx, e, "spx_server_step sync");
@@ -247,7 +310,8 @@ int spx_server_release(spx_server *srv, uint64_t id) {
         hipError_t e = hipStreamSynchronize(nullptr);
         if (e == hipSuccess) e = hipFree(q.film);
         if (e != hipSuccess) return check_hip(srv->ctx, e, "spx_server_release");
-        srv->used -= q.footprint;
+        srv->film_used -= q.film_bytes;
+        srv->queue_used -= q.queue_bytes;
     }
     srv->reqs.erase(it);
     return SPX_OK;

@@ -9,6 +9,7 @@
 // atomics does.  Summation order changes float rounding only.
 #pragma once
 
+#include "spx_bins.hpp"
 #include "spx_device.hpp"
 
 namespace spx {
@@ -38,38 +39,50 @@ struct LdsSplat {
         __hip_atomic_fetch_add(vals + 3 * slot + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    __device__ inline void add(uint32_t idx, f3 v) {
+    // Sums v into idx's slot; false when no slot is free within kProbes.
+    // The occupied-slot counter takes one LDS atomic per wave, not per
+    // inserting lane (all of a wave's inserts would hit that one address).
+    __device__ inline bool try_add(uint32_t idx, f3 v) {
         uint32_t h = (idx * 2654435761u) & (uint32_t)(SLOTS - 1);
+        int slot = -1;
+        bool inserted = false;
         for (int p = 0; p < kProbes; ++p) {
-            uint32_t slot = (h + (uint32_t)p) & (uint32_t)(SLOTS - 1);
-            uint32_t k = keys[slot];
+            uint32_t s = (h + (uint32_t)p) & (uint32_t)(SLOTS - 1);
+            uint32_t k = keys[s];
             if (k == idx) {
-                lds_add(slot, v);
-                return;
+                slot = (int)s;
+                break;
             }
             if (k == kEmptyKey) {
-                uint32_t old = atomicCAS(keys + slot, kEmptyKey, idx);
-                if (old == kEmptyKey) {
-                    atomicAdd(fill, 1u);
-                    lds_add(slot, v);
-                    return;
-                }
-                if (old == idx) {
-                    lds_add(slot, v);
-                    return;
+                uint32_t old = atomicCAS(keys + s, kEmptyKey, idx);
+                if (old == kEmptyKey || old == idx) {
+                    slot = (int)s;
+                    inserted = old == kEmptyKey;
+                    break;
                 }
             }
         }
-        film_add(film, idx, v);
+        const uint64_t ins = __ballot(inserted);
+        if (ins && (int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+            atomicAdd(fill, (uint32_t)__popcll(ins));
+        if (slot < 0) return false;
+        lds_add((uint32_t)slot, v);
+        return true;
     }
 
-    // Whole block must call (contains barriers).  Drains occupied slots.
-    __device__ inline void flush() {
+    __device__ inline void add(uint32_t idx, f3 v) {
+        if (!try_add(idx, v)) film_add(film, idx, v);
+    }
+
+    // Whole block must call (contains barriers).  Hands every occupied slot
+    // to emit(pixel, sum) and empties the table.
+    template <class F>
+    __device__ inline void drain(F emit) {
         __syncthreads();
         for (uint32_t k = threadIdx.x; k < (uint32_t)SLOTS; k += blockDim.x) {
             uint32_t key = keys[k];
             if (key != kEmptyKey) {
-                film_add(film, key, mk(vals[3 * k], vals[3 * k + 1], vals[3 * k + 2]));
+                emit(key, mk(vals[3 * k], vals[3 * k + 1], vals[3 * k + 2]));
                 keys[k] = kEmptyKey;
                 vals[3 * k] = vals[3 * k + 1] = vals[3 * k + 2] = 0.0f;
             }
@@ -78,11 +91,42 @@ struct LdsSplat {
         __syncthreads();
     }
 
+    // Drains to the film with global fp32 atomics.
+    __device__ inline void flush() {
+        drain([&](uint32_t key, f3 v) { film_add(film, key, v); });
+    }
+
     // Whole block must call.  Flushes when more than half the slots are taken.
     __device__ inline void maybe_flush() {
         __syncthreads();
         uint32_t f = *fill;
         __syncthreads();  // every thread has read the same count
+        if (f > (uint32_t)(SLOTS / 2)) flush();
+    }
+};
+
+// The LDS hash in front of binned records (deposit mode 3): deposits on one
+// film pixel within a block are summed in LDS first, so a beam's many
+// deposits on few pixels become few records; the table drains to records,
+// and a deposit that finds no free slot is appended as a record at once.
+// The LDS fp32 sums' order varies between runs (last-bit differences only).
+template <int SLOTS>
+struct HashBinSink {
+    LdsSplat<SLOTS> hash;
+    BinSink bins;
+
+    __device__ inline void add(uint32_t idx, f3 v) {
+        if (!hash.try_add(idx, v)) bins.add(idx, v);
+    }
+    // Whole block must call.
+    __device__ inline void flush() {
+        hash.drain([&](uint32_t key, f3 v) { bins.add(key, v); });
+    }
+    // Whole block must call.  Drains when more than half the slots are taken.
+    __device__ inline void maybe_flush() {
+        __syncthreads();
+        uint32_t f = *hash.fill;
+        __syncthreads();
         if (f > (uint32_t)(SLOTS / 2)) flush();
     }
 };

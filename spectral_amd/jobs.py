@@ -128,13 +128,27 @@ class BatchRunner:
             f.clear(0.0, stream)
 
     def run_batched(self, stream=None, cfg: Optional[A.BatchCfg] = None, subset=None) -> None:
-        if subset is None:
-            tbl, n = self.table, len(self.specs)
-        else:
-            tbl = (A.Job * max(1, len(subset)))(*[self.table[j] for j in subset])
-            n = len(subset)
+        tbl, n = self._table(subset)
         _check(self.ctx, A.lib().spx_batch_run(self.ctx.handle, tbl, n,
                                                C.byref(cfg) if cfg is not None else None, stream))
+
+    def _table(self, subset=None):
+        if subset is None:
+            return self.table, len(self.specs)
+        return (A.Job * max(1, len(subset)))(*[self.table[j] for j in subset]), len(subset)
+
+    def plan(self, stream=None, cfg: Optional[A.BatchCfg] = None, subset=None) -> "BatchPlan":
+        """The table's schedule captured as a hipGraph (spx_batch_plan_create)."""
+        tbl, n = self._table(subset)
+        return BatchPlan(self.ctx, tbl, n, cfg, stream)
+
+    def autotune(self, stream=None, sweeps: int = 1, subset=None):
+        """spx_autotune over the table (or a subset): returns (Tuning, best ms)."""
+        tbl, n = self._table(subset)
+        t, ms = A.Tuning(), C.c_double(0.0)
+        _check(self.ctx, A.lib().spx_autotune(self.ctx.handle, tbl, n, sweeps, stream, C.byref(t),
+                                              C.byref(ms)))
+        return t, ms.value
 
     def run_plain(self, stream=None, engine: int = A.ENGINE_PLAIN) -> None:
         """One launch per job through the single-job entry points."""
@@ -157,3 +171,39 @@ class BatchRunner:
             f.free()
         for s in self.scenes:
             s.close()
+
+
+class BatchPlan:
+    """A job table's launch schedule captured once as a hipGraph; ``launch``
+    replays it, ``update`` rewrites per-job values (RNG bases, cameras, films)
+    that keep the same schedule."""
+
+    def __init__(self, ctx: Context, table, n: int, cfg: Optional[A.BatchCfg] = None, stream=None):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(ctx, A.lib().spx_batch_plan_create(ctx.handle, table, n,
+                                                  C.byref(cfg) if cfg is not None else None, stream,
+                                                  C.byref(h)))
+        self.handle = h
+
+    def update(self, table, n: int, stream=None) -> None:
+        _check(self.ctx, A.lib().spx_batch_plan_update(self.handle, table, n, stream))
+
+    def launch(self, stream=None) -> None:
+        _check(self.ctx, A.lib().spx_batch_plan_launch(self.handle, stream))
+
+    def info(self):
+        r, nodes = C.c_uint32(0), C.c_uint32(0)
+        _check(self.ctx, A.lib().spx_batch_plan_info(self.handle, C.byref(r), C.byref(nodes)))
+        return {"rounds": r.value, "graph_nodes": nodes.value}
+
+    def close(self) -> None:
+        if self.handle:
+            A.lib().spx_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover - best effort
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -4,7 +4,8 @@ A request is K progressive frames of one job drawn from the mixed-job
 distribution (spectral_amd.jobs), its camera orbiting 2 deg per frame.  The
 server (spx_server_*, spx_serve.hip) admits requests by film + replica
 footprint under a memory cap, keeps films and RNG counters resident, and
-runs every active request's next frame in one batched launch per step.
+runs every active request's next frame in one batched launch per step (a
+captured graph, replayed while the set of active requests is unchanged).
 ``run_stream`` replays arrivals in real time and returns each request's
 latency (arrival -> last frame done).
 """
@@ -107,6 +108,15 @@ class Server:
         if s is not None:
             s.close()
 
+    def set_graphs(self, enable: bool) -> None:
+        """Replay captured plans while the active set is unchanged (default on)."""
+        _check(self.ctx, A.lib().spx_server_set_graphs(self.handle, int(bool(enable))))
+
+    def stats(self) -> dict:
+        c, r = C.c_uint64(), C.c_uint64()
+        _check(self.ctx, A.lib().spx_server_stats(self.handle, C.byref(c), C.byref(r)))
+        return {"captures": c.value, "replays": r.value}
+
     def bytes_in_use(self) -> int:
         return A.lib().spx_server_bytes_in_use(self.handle)
 
@@ -184,6 +194,7 @@ def bench_serve(ctx: Context, quick: bool = False, n_req: int = 48) -> dict:
                 "p50_ms": round(float(np.nanpercentile(r.latencies, 50)) * 1e3, 2),
                 "p99_ms": round(float(np.nanpercentile(r.latencies, 99)) * 1e3, 2),
                 "throughput": round(r.units / r.wall_s / 1e6, 1), "steps": r.steps})
+        out["graph_steps"] = srv.stats()
         return out
     finally:
         srv.close()

@@ -59,3 +59,18 @@ def test_null_arguments_rejected_without_device():
     assert lib.spx_camera_accumulate(None, None, None, 1, 1, 1, 0, 0, None, 0, None, None) == A.SPX_EINVAL
     assert lib.spx_batch_run(None, None, 0, None, None) == A.SPX_EINVAL
     assert lib.spx_server_step(None, None, None) == A.SPX_EINVAL
+    assert lib.spx_batch_plan_create(None, None, 0, None, None, None) == A.SPX_EINVAL
+    assert lib.spx_batch_plan_update(None, None, 0, None) == A.SPX_EINVAL
+    assert lib.spx_batch_plan_launch(None, None) == A.SPX_EINVAL
+    assert lib.spx_batch_plan_info(None, None, None) == A.SPX_EINVAL
+    lib.spx_batch_plan_destroy(None)
+    assert lib.spx_autotune(None, None, 0, 1, None, None, None) == A.SPX_EINVAL
+    assert lib.spx_ctx_kernel_times(None, None, None) == A.SPX_EINVAL
+
+
+def test_tuning_layout_matches_header():
+    text = (ROOT / "include" / "spectral_hip.h").read_text()
+    body = text[text.index("typedef struct {\n    uint32_t max_replicas;"):]
+    body = body[:body.index("} spx_tuning;")]
+    fields = re.findall(r"uint32_t (\w+);", body)
+    assert fields == [n for n, _ in A.Tuning._fields_]

@@ -19,7 +19,7 @@ def _small_mix(n=16):
     return jobs.mixed_jobs(n=n, seed=3, max_side=160, max_photons_log2=16, max_spp=12)
 
 
-@pytest.mark.parametrize("splat_mode,bin_log2", [(0, 27), (1, 27), (2, 27), (2, 15)])
+@pytest.mark.parametrize("splat_mode,bin_log2", [(0, 27), (1, 27), (2, 27), (2, 15), (3, 27), (3, 15)])
 def test_batch_equals_alone(ctx, splat_mode, bin_log2):
     # bin_log2 = 15 (32768 records per round) forces many rounds and pool
     # overflow into the atomic fallback.
@@ -43,7 +43,7 @@ def test_batch_equals_alone(ctx, splat_mode, bin_log2):
         br.close()
 
 
-@pytest.mark.parametrize("splat_mode", [1, 2])
+@pytest.mark.parametrize("splat_mode", [1, 2, 3])
 def test_batch_matches_oracle(ctx, oracle, splat_mode):
     specs = _small_mix(8)
     br = jobs.BatchRunner(ctx, specs)

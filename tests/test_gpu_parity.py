@@ -281,12 +281,12 @@ def test_invalid_arguments_fail_loudly(ctx):
     assert tr.sum().sum() > 0
 
 
-@pytest.mark.parametrize("splat_mode,bin_log2", [(1, 30), (2, 30), (2, 15)])
+@pytest.mark.parametrize("splat_mode,bin_log2", [(1, 30), (2, 30), (2, 15), (3, 30), (3, 15)])
 def test_deposit_modes_wavefront(ctx, oracle, splat_mode, bin_log2):
     # splat mode 1: LDS hash in front of film atomics; mode 2: deposits as
     # records, sorted by 32x32 tile and accumulated in LDS as fixed point (no
-    # film atomics); bin_log2 = 15 overflows the pools into the atomic
-    # fallback.
+    # film atomics); mode 3: the LDS hash draining to those records;
+    # bin_log2 = 15 overflows the pools into the atomic fallback.
     old = ctx.tuning()
     ctx.set_tuning(splat_mode=splat_mode, bin_log2=bin_log2)
     try:

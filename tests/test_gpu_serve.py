@@ -135,3 +135,28 @@ def test_poisson_stream_latencies(ctx):
         assert res.steps >= max(k for _, k in reqs)
     finally:
         srv.close()
+
+
+def test_graph_replays_equal_eager_steps(ctx):
+    # Requests of equal length keep the active set unchanged: the first step
+    # captures a plan, the others replay it with advanced RNG bases.
+    specs = [(jobs.JobSpec(A.JOB_CAMERA, 40, spp=2, seed=21), 4, 0.0),
+             (jobs.JobSpec(A.JOB_VOLUME, 56, n_photons=7000, seed=22, rotation=30.0), 4, 2.0),
+             (jobs.JobSpec(A.JOB_CAMERA, 48, spp=3, seed=23), 4, 2.0)]
+    films, stats = [], []
+    for graphs in (True, False):
+        srv = serve.Server(ctx)
+        try:
+            srv.set_graphs(graphs)
+            ids = [srv.submit(srv.request(*s)) for s in specs]
+            while any(srv.info(i).state != A.REQ_DONE for i in ids):
+                srv.step()
+            films.append([srv.film(i, s.size, s.size) for i, (s, _, _) in zip(ids, specs)])
+            stats.append(srv.stats())
+        finally:
+            srv.close()
+    assert stats[0] == {"captures": 1, "replays": 3}, stats[0]
+    assert stats[1] == {"captures": 0, "replays": 0}, stats[1]
+    for g, e in zip(*films):
+        assert g.sum() > 0
+        assert (rel_l1_per_channel(g, e) < ALONE_TOL).all()
