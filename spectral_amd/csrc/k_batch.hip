@@ -44,7 +44,7 @@ __device__ inline void batch_camera_item(const SceneData &S, const float4 *tbl, 
             bool alive = camera_bounce(S, tbl, P, b, st);
             ++b;
             if (alive && b < P.max_depth) continue;
-            f3 c = camera_xyz(tbl, st);
+            f3 c = camera_xyz(tbl, st, P.illuminant == SPX_ILLUM_E);
             __hip_atomic_fetch_add(acc + 3 * lp + 0, c.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_fetch_add(acc + 3 * lp + 1, c.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             __hip_atomic_fetch_add(acc + 3 * lp + 2, c.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -71,6 +71,7 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
     if (f >= count) return;
     PhotonState st;
     photon_begin(P, tbl, P.photon_base + start + f, st);
+    sink_photon_begin(sink);
     uint32_t b = 0;
     while (true) {
         bool alive;
@@ -82,6 +83,7 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
         f += blockDim.x;
         if (f >= count) break;
         photon_begin(P, tbl, P.photon_base + start + f, st);
+        sink_photon_begin(sink);
         b = 0;
     }
 }

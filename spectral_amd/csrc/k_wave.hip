@@ -62,7 +62,7 @@ __device__ inline uint32_t cam_load(const CamQueue &q, uint32_t i, CamState &st)
     float t0 = q.tp[i];
     st.tp[0] = t0;
     st.tp[1] = st.tp[2] = st.tp[3] = collapsed ? 0.0f : t0;
-    st.rad[0] = st.rad[1] = st.rad[2] = st.rad[3] = 0.0f;
+    st.esc = 0.0f;
     st.valid = collapsed ? 1u : (uint32_t)kHeroN;
     return s & 0x7FFFFFFFu;
 }
@@ -73,8 +73,9 @@ struct CamWave {
     uint32_t *counts;      // [33] live counts per bounce
 };
 
-__device__ inline void cam_finish(const CamWave &W, const float4 *tbl, const CamState &st, uint32_t slot) {
-    f3 c = camera_xyz(tbl, st);
+__device__ inline void cam_finish(const CamWave &W, const float4 *tbl, const CamState &st, uint32_t slot,
+                                  bool equal_energy) {
+    f3 c = camera_xyz(tbl, st, equal_energy);
     W.rx[slot] = c.x;
     W.ry[slot] = c.y;
     W.rz[slot] = c.z;
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(256) void k_cam_first(CamParams P, CamWave W, uint3
         bool alive = camera_bounce(S, tbl, P, 0, st) && P.max_depth > 1;
         uint32_t at = wave_enqueue(alive, W.counts + 1);
         if (alive) cam_store(W.q[1], at, st, j);
-        else cam_finish(W, tbl, st, j);
+        else cam_finish(W, tbl, st, j, P.illuminant == SPX_ILLUM_E);
     }
 }
 
@@ -118,7 +119,7 @@ __global__ __launch_bounds__(256) void k_cam_bounce(CamParams P, CamWave W, uint
         bool alive = camera_bounce(S, tbl, P, b, st) && (b + 1u < P.max_depth);
         uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
         if (alive) cam_store(qout, at, st, slot);
-        else cam_finish(W, tbl, st, slot);
+        else cam_finish(W, tbl, st, slot, P.illuminant == SPX_ILLUM_E);
     }
 }
 

@@ -67,12 +67,13 @@ constexpr float kLambdaRange = kLambdaMax - kLambdaMin;
 constexpr int kHeroN = 4;
 constexpr float kHeroStep = kLambdaRange / (float)kHeroN;
 
-__device__ inline float rem_euclid(float a, float b) {
-    float r = fmodf(a, b);
-    return r < 0.0f ? r + fabsf(b) : r;
-}
+// from_hero_u's rem_euclid(u * 350 + j * 87.5, 350) for u in [0, 1) and j in
+// 0..3: the argument lies in [0, 612.5], below twice the modulus, where fmod
+// is one subtraction and that subtraction is exact (Sterbenz: b <= a < 2b),
+// so the result equals the reference's bit for bit without fmodf's loop.
 __device__ inline float hero_lambda(float u, int j) {
-    return kLambdaMin + rem_euclid(u * kLambdaRange + (float)j * kHeroStep, kLambdaRange);
+    float a = u * kLambdaRange + (float)j * kHeroStep;
+    return kLambdaMin + (a >= kLambdaRange ? a - kLambdaRange : a);
 }
 
 // ---------------------------------------------------------------------------
@@ -116,21 +117,24 @@ __device__ inline TblPos tbl_pos(float nm) {
     return TblPos{i, f - (float)i, i + 1 >= kTblN};
 }
 // Table::sample on the three CMF columns.
-__device__ inline f3 cmf_lookup(const float4 *tbl, float nm) {
-    TblPos p = tbl_pos(nm);
+__device__ inline f3 cmf_at(const float4 *tbl, TblPos p) {
     float4 a = tbl[p.i];
     if (p.last) return f3{a.x, a.y, a.z};
     float4 b = tbl[p.i + 1];
     return f3{a.x + (b.x - a.x) * p.frac, a.y + (b.y - a.y) * p.frac, a.z + (b.z - a.z) * p.frac};
 }
+__device__ inline f3 cmf_lookup(const float4 *tbl, float nm) { return cmf_at(tbl, tbl_pos(nm)); }
 // Table::sample on the illuminant column (1.0 for equal energy).
-__device__ inline float ill_lookup(const float4 *tbl, float nm, bool equal_energy) {
+__device__ inline float ill_at(const float4 *tbl, TblPos p, bool equal_energy) {
     if (equal_energy) return 1.0f;
-    TblPos p = tbl_pos(nm);
     float a = tbl[p.i].w;
     if (p.last) return a;
     float b = tbl[p.i + 1].w;
     return a + (b - a) * p.frac;
+}
+__device__ inline float ill_lookup(const float4 *tbl, float nm, bool equal_energy) {
+    if (equal_energy) return 1.0f;
+    return ill_at(tbl, tbl_pos(nm), false);
 }
 
 // ---------------------------------------------------------------------------

@@ -21,13 +21,17 @@ __device__ inline void stage_scene(SceneData &dst, const SceneData *src, float4 
 // Russian roulette before bounce b >= rr_depth: survive with
 // q = clamp(max throughput, 0.05, 0.95), throughput /= q).
 // ---------------------------------------------------------------------------
+// A path's radiance is added once, at its escape (tracer.rs:63-70), as
+// tp[k] * bg * S(lambda_k) onto zero; the state keeps bg (zero until the
+// path escapes) and camera_xyz forms each lane's radiance with the same
+// operations, so the CMF and illuminant lookups share one table position.
 struct CamState {
     f3 ro, rd;
     uint32_t rng;
     float lam0;     // hero wavelength (lane 0)
     float u_hero;   // the comb draw; lane k is hero_lambda(u_hero, k)
     float tp[4];
-    float rad[4];
+    float esc;      // background radiance at the escape (0 = not escaped)
     uint32_t valid;
 };
 
@@ -40,10 +44,8 @@ __device__ inline void camera_begin(const CamParams &P, uint32_t px, uint32_t py
     st.u_hero = next_f32(st.rng);
     st.lam0 = hero_lambda(st.u_hero, 0);
 #pragma unroll
-    for (int k = 0; k < kHeroN; ++k) {
-        st.tp[k] = 1.0f;
-        st.rad[k] = 0.0f;
-    }
+    for (int k = 0; k < kHeroN; ++k) st.tp[k] = 1.0f;
+    st.esc = 0.0f;
     st.valid = kHeroN;
     st.ro = ld3(P.cam.origin);
     st.rd = primary_ray(P.cam, s, t);
@@ -62,13 +64,7 @@ __device__ inline bool camera_bounce(const SceneData &S, const float4 *tbl, cons
     Hit h;
     int mi = scene_hit(S, st.ro, st.rd, h);
     if (mi < 0) {
-        float bg = background_radiance(S.env, st.rd);
-        bool eq = P.illuminant == SPX_ILLUM_E;
-#pragma unroll
-        for (int k = 0; k < kHeroN; ++k) {
-            float sp = ill_lookup(tbl, hero_lambda(st.u_hero, k), eq);
-            st.rad[k] += st.tp[k] * bg * sp;
-        }
+        st.esc = background_radiance(S.env, st.rd);
         return false;
     }
     const spx_material &m = S.mats[mi];
@@ -87,14 +83,20 @@ __device__ inline bool camera_bounce(const SceneData &S, const float4 *tbl, cons
     return true;
 }
 
-// XYZ of a finished path, tracer.rs:102-110.
-__device__ inline f3 camera_xyz(const float4 *tbl, const CamState &st) {
+// XYZ of a finished path, tracer.rs:102-110.  Lane k's radiance is
+// 0 + tp[k] * bg * S(lambda_k) (tracer.rs:66-68; exactly tp[k] * bg * S, as
+// the product is never -0), and dividing by valid_lanes, 1 or 4, is exact
+// scaling by 1 or 0.25.
+__device__ inline f3 camera_xyz(const float4 *tbl, const CamState &st, bool equal_energy) {
     const float pdf = 1.0f / kLambdaRange;
+    const float inv_valid = st.valid == 1 ? 1.0f : 0.25f;
     f3 xyz = mk(0.f, 0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < kHeroN; ++k) {
-        f3 c = cmf_lookup(tbl, hero_lambda(st.u_hero, k));
-        float wk = st.rad[k] / pdf / (float)st.valid;
+        const TblPos p = tbl_pos(hero_lambda(st.u_hero, k));
+        const f3 c = cmf_at(tbl, p);
+        const float rad = st.tp[k] * st.esc * ill_at(tbl, p, equal_energy);
+        const float wk = rad / pdf * inv_valid;
         xyz.x += c.x * wk;
         xyz.y += c.y * wk;
         xyz.z += c.z * wk;
@@ -108,7 +110,7 @@ __device__ inline f3 camera_path(const SceneData &S, const float4 *tbl, const Ca
     camera_begin(P, px, py, sidx, st);
     for (uint32_t b = 0; b < P.max_depth; ++b)
         if (!camera_bounce(S, tbl, P, b, st)) break;
-    return camera_xyz(tbl, st);
+    return camera_xyz(tbl, st, P.illuminant == SPX_ILLUM_E);
 }
 
 // ---------------------------------------------------------------------------
@@ -131,6 +133,13 @@ __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, ui
     st.rd = ld3(P.beam.dir);
     st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
 }
+
+// Hooks a deposit sink may take: a new photon starts, the photon scattered
+// off a material of `kind`.  Sinks that track photons overload these.
+template <class Sink>
+__device__ inline void sink_photon_begin(Sink &) {}
+template <class Sink>
+__device__ inline void sink_scattered(Sink &, uint32_t) {}
 
 // Scatter at a hit (the tail of every photon loop).  Returns tir.
 __device__ inline bool photon_scatter(const SceneData &S, const PhotonParams &P, int mi, const Hit &h,
@@ -206,6 +215,7 @@ __device__ inline int lighttrace_bounce(const SceneData &S, const float4 *tbl, c
     }
     if (mi < 0) return 2;
     photon_scatter(S, P, mi, h, st);
+    sink_scattered(sink, S.mats[mi].kind);
     return 0;
 }
 
@@ -341,6 +351,7 @@ __device__ inline bool volume_bounce(const SceneData &S, const float4 *tbl, cons
     else volume_beam_segment(P, st, cmf, seg_len, sink);
     if (mi < 0) return false;
     photon_scatter(S, P, mi, h, st);
+    sink_scattered(sink, S.mats[mi].kind);
     return true;
 }
 

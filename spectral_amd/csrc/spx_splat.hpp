@@ -109,14 +109,17 @@ struct LdsSplat {
 // film pixel within a block are summed in LDS first, so a beam's many
 // deposits on few pixels become few records; the table drains to records,
 // and a deposit that finds no free slot is appended as a record at once.
+// Once a photon has scattered off a diffuse surface its deposits spread
+// over the film and rarely meet another's, so they skip the table.
 // The LDS fp32 sums' order varies between runs (last-bit differences only).
 template <int SLOTS>
 struct HashBinSink {
     LdsSplat<SLOTS> hash;
     BinSink bins;
+    bool spread = false;  // this lane's photon has scattered diffusely
 
     __device__ inline void add(uint32_t idx, f3 v) {
-        if (!hash.try_add(idx, v)) bins.add(idx, v);
+        if (spread || !hash.try_add(idx, v)) bins.add(idx, v);
     }
     // Whole block must call.
     __device__ inline void flush() {
@@ -130,5 +133,14 @@ struct HashBinSink {
         if (f > (uint32_t)(SLOTS / 2)) flush();
     }
 };
+
+template <int SLOTS>
+__device__ inline void sink_photon_begin(HashBinSink<SLOTS> &s) {
+    s.spread = false;
+}
+template <int SLOTS>
+__device__ inline void sink_scattered(HashBinSink<SLOTS> &s, uint32_t kind) {
+    if (kind == SPX_MAT_LAMBERTIAN) s.spread = true;
+}
 
 }  // namespace spx
