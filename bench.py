@@ -71,18 +71,22 @@ class Timer:
         return ms / reps, (time.perf_counter() - t0) * 1e3 / reps
 
 
-def profiled_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` on this workload, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, copy-calibrated), or
-    None when the summary has no figure for it."""
-    path = ROOT / "profiles" / "batched_step_mixed512_hbm_counters.md"
+def profile_figure(summary: str, key: str):
+    """A `key: value` figure of a committed rocprofv3 summary under profiles/
+    (None when the summary has no figure for it)."""
     try:
-        for line in path.read_text().splitlines():
-            if line.startswith(f"traffic_bytes_per_launch[{kernel}]:"):
+        for line in (ROOT / "profiles" / summary).read_text().splitlines():
+            if line.startswith(f"{key}:"):
                 return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload (FETCH_SIZE and
+    WRITE_SIZE, copy-calibrated)."""
+    return profile_figure("batched_step_mixed512_hbm_counters.md", f"traffic_bytes_per_launch[{kernel}]")
 
 
 def cpu_baseline(paths_total, photons_total, target_s=12.0):
@@ -189,10 +193,16 @@ def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
     per_launch = alg_bytes_step / max(launches_step, 1)
     ms_launch = ms_step / max(launches_step, 1)
     achieved = per_launch / (ms_launch / 1e3) / 1e9 if ms_launch > 0 else 0.0
-    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(name), "kernel": name,
-            "kernel_ms_per_launch": round(ms_launch, 3), "launches_per_step": launches_step,
-            "alg_bytes_per_launch": round(per_launch)}
+    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(name), "kernel": name,
+           "kernel_ms_per_launch": round(ms_launch, 3), "launches_per_step": launches_step,
+           "alg_bytes_per_launch": round(per_launch)}
+    # What actually bounds it: the share of SIMD cycles issuing VALU work
+    # (SQ counters of the same workload).
+    busy = profile_figure("batched_step_mixed512_sq_counters.md", f"valu_busy[{name}]")
+    if busy is not None:
+        out["valu_busy"] = busy
+    return out
 
 
 def main() -> int:
@@ -294,7 +304,11 @@ def main() -> int:
     dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
     roofline = dict(groups[dominant])
     roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
-                                                 "frac", "traffic")} for g, v in groups.items()}
+                                                 "frac", "traffic", "valu_busy") if k in v}
+                           for g, v in groups.items()}
+    roofline["note"] = ("achieved = algorithmic bytes (wavefront path state + deposits) / kernel time; the "
+                        "persistent kernels keep path state in registers, so measured HBM traffic is the "
+                        "smaller `traffic` and the bound is VALU issue (`valu_busy`)")
     roofline["work_model"] = {k: round(v, 4) for k, v in wm.items()}
     roofline["work_model_source"] = "oracle counts on the cpu_baseline samples" if cpu else "pinned constants"
     result = {
