@@ -139,24 +139,47 @@ def cpu_baseline(paths_total, photons_total, target_s=12.0):
 
 
 def per_config(ctx, timer, quick):
-    """Throughput of each single-job configuration (best engine listed first)."""
+    """Throughput of each single-job configuration under every engine: plain
+    (one thread per path / photon), wavefront (per-bounce queues) and the
+    batched engine running the one job (spx_batch_run)."""
+    import ctypes as C
+
     import spectral_amd as S
     from spectral_amd import _abi as A
     from spectral_amd import scenes
+    from spectral_amd.core import _check, opts
     out = {}
     st = timer.stream
-    # DSOTM prism, 2^20 photons, 320x180
+
+    def batched(job):
+        tbl = (A.Job * 1)(job)
+        return lambda: _check(ctx, A.lib().spx_batch_run(ctx.handle, tbl, 1, None, st))
+
+    def volume_job(ds, cam, bm, w, h, n, ill, disp, film, zbuf=None):
+        j = A.Job()
+        j.kind, j.width, j.height, j.n_photons, j.seed = A.JOB_VOLUME, w, h, n, 0
+        j.estimator = A.EST_POINT
+        j.opts = opts(ill, disp)
+        j.camera, j.beam, j.vol = cam.basis, bm, A.VolParams(**scenes.HAZE)
+        j.scene, j.film = ds.handle.value, film.ptr
+        j.zbuf = zbuf.ptr if zbuf is not None else None
+        return j
+
+    # C0: DSOTM prism, 2^20 photons, 320x180
     sc = scenes.prism_scene().upload(ctx)
     bm, cam = scenes.prism_beam(), scenes.prism_camera()
     film = S.Film(ctx, 320, 180)
-    for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront")):
-        f = lambda: S.render_volume(ctx, sc, cam, bm, 320, 180, 1 << 20, 0.05, 0.05, 0.0, 14.0, None,
-                                    0, film=film, illuminant=A.ILLUM_D65, engine=eng, stream=st)
+    forms = [(nm, (lambda eng=eng: S.render_volume(ctx, sc, cam, bm, 320, 180, 1 << 20, 0.05, 0.05, 0.0, 14.0,
+                                                   None, 0, film=film, illuminant=A.ILLUM_D65, engine=eng,
+                                                   stream=st)))
+             for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront"))]
+    forms.append(("batched", batched(volume_job(sc, cam, bm, 320, 180, 1 << 20, A.ILLUM_D65, True, film))))
+    for nm, f in forms:
         f()
         ms, _ = timer.run(f, 3)
         out[f"prism_2^20_320x180_{nm}_mphotons_s"] = round((1 << 20) / ms / 1e3, 1)
     film.free()
-    # rainbow 512^2 x 64 spp
+    # C1: rainbow 512^2 x 64 spp
     rs, rc = scenes.rainbow_scene().upload(ctx), scenes.rainbow_camera(512, 512)
     for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront")):
         tr = S.HipTracer(ctx, rs, rc, 512, 512, A.ILLUM_D65, 1, engine=eng, rr_depth=3)
@@ -165,26 +188,44 @@ def per_config(ctx, timer, quick):
         ms, _ = timer.run(f, 2)
         out[f"rainbow_512x512x64_{nm}_mpaths_s"] = round(512 * 512 * 64 / ms / 1e3, 1)
         tr.film.free()
-    # full-size prism on floor, 1920x1080
+    film = S.Film(ctx, 512, 512)
+    j = A.Job()
+    j.kind, j.width, j.height, j.spp, j.seed = A.JOB_CAMERA, 512, 512, 64, 1
+    j.opts, j.camera, j.scene, j.film = opts(A.ILLUM_D65, True, 8, 3), rc.basis, rs.handle.value, film.ptr
+    f = batched(j)
+    f()
+    ms, _ = timer.run(f, 2)
+    out["rainbow_512x512x64_batched_mpaths_s"] = round(512 * 512 * 64 / ms / 1e3, 1)
+    film.free()
+    # C2: full-size prism on floor, 1920x1080 with the surface z-buffer, and
+    # its floor caustic on a 1024x1024 screen
     n = 1 << (24 if quick else 28)
     fs = scenes.prism_scene(floor=True).upload(ctx)
     ps = scenes.prism_scene().upload(ctx)
     fc = scenes.floor_camera(1920, 1080)
+    rim, zbuf = S.surface_rim(ctx, fs, fc, 1920, 1080)
+    rim.free()
     film = S.Film(ctx, 1920, 1080)
     scr = scenes.floor_screen(1024, 1024)
     sfilm = S.Film(ctx, 1024, 1024)
+    tag = f"2^{n.bit_length() - 1}"
     for label, ill, disp in (("D65", A.ILLUM_D65, True), ("A", A.ILLUM_A, True), ("disp_off", A.ILLUM_D65, False)):
-        f = lambda: S.render_volume(ctx, fs, fc, bm, 1920, 1080, n, 0.05, 0.05, 0.0, 14.0, None, 0,
+        f = lambda: S.render_volume(ctx, fs, fc, bm, 1920, 1080, n, 0.05, 0.05, 0.0, 14.0, zbuf, 0,
                                     film=film, illuminant=ill, dispersion=disp, engine=A.ENGINE_PLAIN,
                                     stream=st)
         ms, _ = timer.run(f, 1)
-        out[f"floor_volume_2^{n.bit_length() - 1}_1920x1080_{label}_mphotons_s"] = round(n / ms / 1e3, 1)
-        g = lambda: S.lighttrace_trace(ctx, ps, scr, bm, n, 0x0F1, film=sfilm, illuminant=ill,
-                                       dispersion=disp, stream=st)
-        ms, _ = timer.run(g, 1)
-        out[f"floor_caustic_2^{n.bit_length() - 1}_{label}_mphotons_s"] = round(n / ms / 1e3, 1)
+        out[f"floor_volume_{tag}_1920x1080_{label}_plain_mphotons_s"] = round(n / ms / 1e3, 1)
+        f = batched(volume_job(fs, fc, bm, 1920, 1080, n, ill, disp, film, zbuf))
+        ms, _ = timer.run(f, 1)
+        out[f"floor_volume_{tag}_1920x1080_{label}_batched_mphotons_s"] = round(n / ms / 1e3, 1)
+        for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront")):
+            g = lambda: S.lighttrace_trace(ctx, ps, scr, bm, n, 0x0F1, film=sfilm, illuminant=ill,
+                                           dispersion=disp, engine=eng, stream=st)
+            ms, _ = timer.run(g, 1)
+            out[f"floor_caustic_{tag}_{label}_{nm}_mphotons_s"] = round(n / ms / 1e3, 1)
     film.free()
     sfilm.free()
+    zbuf.free()
     return out
 
 
