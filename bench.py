@@ -246,6 +246,20 @@ def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
     return out
 
 
+def atomic_peak(ctx, timer, film_floats=3 * 1920 * 1080):
+    """A_peak: scattered fp32 atomicAdd throughput (G ops/s) on a buffer of
+    the largest film's size, addresses uniform over it (spx_atomic_probe)."""
+    import spectral_amd as S
+    from spectral_amd import _abi as A
+    buf = S.DeviceBuffer(ctx, film_floats)
+    threads, ops = 1 << 20, 64
+    f = lambda: A.lib().spx_atomic_probe(ctx.handle, buf.ptr, film_floats, ops, threads, timer.stream)
+    f()
+    ms, _ = timer.run(f, 3)
+    buf.free()
+    return threads * ops / ms / 1e6
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -342,11 +356,22 @@ def main() -> int:
     }
     if kt["bins"][1]:
         groups["bins"] = kernel_roofline("k_bins passes", dep_bytes, kt["bins"][0] / steps, kt["bins"][1] // steps)
+    # SURVEY 8(d)'s second ceiling: the deposits as fp32 atomics (3 per
+    # deposit) over the photon kernel's time, against the measured A_peak.
+    a_peak = atomic_peak(ctx, timer)
+    ph_ms = groups["photon"]["kernel_ms_per_launch"] * groups["photon"]["launches_per_step"]
+    dep_atomics = photons * wm["splat_per_photon"] * 3
+    for g in groups.values():
+        g["atomic_peak_gops"] = round(a_peak, 2)
+    groups["photon"]["atomics_gops"] = round(dep_atomics / (ph_ms / 1e3) / 1e9, 2)
+    groups["photon"]["atomic_frac"] = round(groups["photon"]["atomics_gops"] / a_peak, 3)
     dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
     roofline = dict(groups[dominant])
     roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
-                                                 "frac", "traffic", "valu_busy") if k in v}
+                                                 "frac", "traffic", "valu_busy", "atomics_gops", "atomic_frac")
+                               if k in v}
                            for g, v in groups.items()}
+    roofline["atomic_peak_gops"] = round(a_peak, 2)
     roofline["note"] = ("achieved = algorithmic bytes (wavefront path state + deposits) / kernel time; the "
                         "persistent kernels keep path state in registers, so measured HBM traffic is the "
                         "smaller `traffic` and the bound is VALU issue (`valu_busy`)")

@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
     __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kBatchSlots : 1];
     __shared__ uint32_t sfill;
     __shared__ uint32_t wstate[2 * 4];  // binned deposits: {chunk, fill} per wave
-    LdsSplat<kBatchSlots> hash{sk, sv, &sfill, nullptr};
+    LdsSplat<kBatchSlots, SPLAT == 3 ? kHashBinProbes : 8> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
         B.scene = nullptr;
         B.tbl_src = nullptr;
@@ -183,15 +183,15 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
         const uint32_t kind = uniform_u32(jobs[j].kind);
         const PhotonParams P = jobs[j].ph;
         stage_for(B, P.scene, P.tbl);
-        if (SPLAT == 0) {
+        if constexpr (SPLAT == 0) {
             GlobalSink g{deposit_film(P)};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
-        } else if (SPLAT == 2) {
+        } else if constexpr (SPLAT == 2) {
             const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
             BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
                       wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
-        } else if (SPLAT == 3) {
+        } else if constexpr (SPLAT == 3) {
             const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
             HashBinSink<kBatchSlots> hb{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
                                                       pool, P.key_base, P.film, wstate, fw,

@@ -294,7 +294,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
         step(sink);
         sink.flush();
     } else if constexpr (SINK == 3) {
-        LdsSplat<kSplatSlots> hash{L.sk, L.sv, &L.sfill, nullptr};
+        LdsSplat<kSplatSlots, kHashBinProbes> hash{L.sk, L.sv, &L.sfill, nullptr};
         hash.init();
         if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
         __syncthreads();

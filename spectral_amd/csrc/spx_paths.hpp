@@ -219,12 +219,28 @@ __device__ inline int lighttrace_bounce(const SceneData &S, const float4 *tbl, c
     return 0;
 }
 
-// Point / legacy estimator deposits for one segment (volume.rs:130-150, 51-64).
-template <bool LEGACY, class Sink>
-__device__ inline void volume_point_segment(const PhotonParams &P, const float4 *tbl,
-                                            const PhotonState &st, f3 cmf, float seg_len,
-                                            uint32_t &rng, Sink &sink) {
+// A job-wide value known to be the same on every lane, moved to a scalar
+// register so that branches on it are scalar branches: the compiler cannot
+// prove a value read from the job table uniform, and would otherwise
+// if-convert such a branch and evaluate both sides on every lane.
+__device__ inline float uniform_f32(float v) {
+    return __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(v)));
+}
+__device__ inline const float *uniform_ptr(const float *p) {
+    const uint64_t a = (uint64_t)(uintptr_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+    return (const float *)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Point / legacy estimator deposits for one segment (volume.rs:130-150,
+// 51-64).  ISO: g == 0, where phase_hg(0, c) = 1 / (4 pi * 1^1.5) for every
+// c, bit for bit.  Without a z-buffer every depth passes (+INF).
+template <bool LEGACY, bool ISO, class Sink>
+__device__ inline void point_deposits(const PhotonParams &P, const PhotonState &st, f3 cmf, float seg_len,
+                                      uint32_t &rng, Sink &sink) {
     const f3 cam_o = ld3(P.proj.o);
+    const float *zbuf = uniform_ptr(P.zbuf);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         float dist = next_f32(rng) * seg_len;
@@ -239,16 +255,22 @@ __device__ inline void volume_point_segment(const PhotonParams &P, const float4 
         } else {
             f3 to = cam_o - p;
             float d = fmaxf(length(to), 1e-3f);
-            float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
-            if (d < zb) {
-                // g == 0: phase_hg(0, c) = 1 / (4 pi * 1^1.5) for every c, bit for bit.
-                float phase = P.vp.g == 0.0f ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
-                float trans = expf(-P.vp.sigma_t * d);
-                float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
-                sink.add(idx, cmf * contrib);
-            }
+            if (zbuf && !(d < zbuf[idx])) continue;
+            float phase = ISO ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
+            float trans = expf(-P.vp.sigma_t * d);
+            float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
+            sink.add(idx, cmf * contrib);
         }
     }
+}
+
+template <bool LEGACY, class Sink>
+__device__ inline void volume_point_segment(const PhotonParams &P, const float4 *tbl,
+                                            const PhotonState &st, f3 cmf, float seg_len,
+                                            uint32_t &rng, Sink &sink) {
+    if (LEGACY) point_deposits<true, true>(P, st, cmf, seg_len, rng, sink);
+    else if (uniform_f32(P.vp.g) == 0.0f) point_deposits<false, true>(P, st, cmf, seg_len, rng, sink);
+    else point_deposits<false, false>(P, st, cmf, seg_len, rng, sink);
 }
 
 // Beam estimator deposits for one segment (volume.rs:263-311, 178-218).

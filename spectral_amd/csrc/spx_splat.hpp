@@ -4,7 +4,7 @@
 // Deposits that land on the same pixel within the block are combined with
 // LDS fp32 atomics (ds_add_f32); the table is drained to the film with
 // global fp32 atomics (global_atomic_add_f32) when it is flushed.  A deposit
-// that finds no free slot within kProbes goes straight to the film, so the
+// that finds no free slot within its probes goes straight to the film, so the
 // result never depends on the table size -- only the number of global
 // atomics does.  Summation order changes float rounding only.
 #pragma once
@@ -15,9 +15,13 @@
 namespace spx {
 
 constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
-constexpr int kProbes = 8;
 
-template <int SLOTS>
+// PROBES: linear-probe length.  Draining to film atomics, a miss costs a
+// global atomic, so the table probes far (8); draining to records, a miss
+// costs only a record, and short probing is cheaper overall (2 measured
+// fastest for the batched photon kernel: fewer divergent compare-and-swap
+// rounds per deposit, slightly more records).
+template <int SLOTS, int PROBES = 8>
 struct LdsSplat {
     static_assert((SLOTS & (SLOTS - 1)) == 0, "power of two");
     uint32_t *keys;  // [SLOTS]
@@ -39,14 +43,14 @@ struct LdsSplat {
         __hip_atomic_fetch_add(vals + 3 * slot + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
 
-    // Sums v into idx's slot; false when no slot is free within kProbes.
+    // Sums v into idx's slot; false when no slot is free within PROBES.
     // The occupied-slot counter takes one LDS atomic per wave, not per
     // inserting lane (all of a wave's inserts would hit that one address).
     __device__ inline bool try_add(uint32_t idx, f3 v) {
         uint32_t h = (idx * 2654435761u) & (uint32_t)(SLOTS - 1);
         int slot = -1;
         bool inserted = false;
-        for (int p = 0; p < kProbes; ++p) {
+        for (int p = 0; p < PROBES; ++p) {
             uint32_t s = (h + (uint32_t)p) & (uint32_t)(SLOTS - 1);
             uint32_t k = keys[s];
             if (k == idx) {
@@ -112,9 +116,11 @@ struct LdsSplat {
 // Once a photon has scattered off a diffuse surface its deposits spread
 // over the film and rarely meet another's, so they skip the table.
 // The LDS fp32 sums' order varies between runs (last-bit differences only).
+constexpr int kHashBinProbes = 2;
+
 template <int SLOTS>
 struct HashBinSink {
-    LdsSplat<SLOTS> hash;
+    LdsSplat<SLOTS, kHashBinProbes> hash;
     BinSink bins;
     bool spread = false;  // this lane's photon has scattered diffusely
 
