@@ -173,6 +173,9 @@ typedef struct {
                                 (0 = 4 per compute unit) */
     uint32_t chunk_paths;    /* batched engine: camera work item, in paths (default 16384) */
     uint32_t chunk_photons;  /* batched engine: photon work item, in photons (default 8192) */
+    uint32_t overlap_bins;   /* batched engine, binned deposits: 1 = double-buffered rounds, the bin
+                                passes of round r on a second stream while round r+1 traces
+                                (two record buffers); default 0 */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */

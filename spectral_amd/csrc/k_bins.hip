@@ -417,7 +417,7 @@ __global__ __launch_bounds__(256) void k_bins_accumulate(BinState st, const uint
         const uint32_t height = npix / width;
         const uint32_t tiles_x = tiles_across(width, st.tile_wlog2);
         float *film = st.job_film[job];
-        const bool exclusive = nparts == 1 && !job_shared[job];
+        const bool exclusive = nparts == 1 && !job_shared[job] && !st.atomic_film;
         if (fixed && nparts > 1) {  // into the strip's accumulator; written by k_bins_macc_flush
             unsigned long long *dst = st.macc + (size_t)macc_i * 3 * kStripPix;
             for (uint32_t k = threadIdx.x; k < 3 * kStripPix; k += blockDim.x)
@@ -470,7 +470,7 @@ __global__ __launch_bounds__(256) void k_bins_macc_flush(BinState st, const uint
         const float x = from_fixed(a[0], e), y = from_fixed(a[1], e), z = from_fixed(a[2], e);
         float *film = st.job_film[job];
         const uint32_t pix = py * width + px;
-        if (job_shared[job]) {
+        if (job_shared[job] || st.atomic_film) {
             film_add(film, pix, mk(x, y, z));
         } else {
             film[3 * pix + 0] += x;

@@ -196,6 +196,7 @@ void bins_bind(char *b, const BinLayout &L, const BinJobMap &map, BinState &st, 
     st.pool_cap = (uint32_t)(L.cap / kBinPools);
     st.n_strips = (uint32_t)n;
     st.tile_wlog2 = map.tile_wlog2;
+    st.atomic_film = 0;
     *job_shared = (const uint32_t *)(b + L.o_map + L.m_sh);
 }
 
@@ -212,21 +213,23 @@ int bins_upload_map(spx_ctx *ctx, char *b, const BinLayout &L, const BinJobMap &
 }
 
 int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
-                 const uint32_t **job_shared) {
+                 const uint32_t **job_shared, int slot) {
     const BinLayout L = bins_layout(map, want, ctx->tuning.bin_log2);
-    if (ctx->bins_bytes < L.total) {
+    void *&buf = ctx->bins[slot];
+    size_t &bytes = ctx->bins_bytes[slot];
+    if (bytes < L.total) {
         hipError_t e = hipDeviceSynchronize();  // earlier rounds may still use the old buffers
         if (e != hipSuccess) return check_hip(ctx, e, "prepare_bins sync");
-        if (ctx->bins) hipFree(ctx->bins);
-        ctx->bins = nullptr;
-        ctx->bins_bytes = 0;
+        if (buf) hipFree(buf);
+        buf = nullptr;
+        bytes = 0;
         size_t want_bytes = L.total + L.total / 8;
-        e = hipMalloc(&ctx->bins, want_bytes);
+        e = hipMalloc(&buf, want_bytes);
         ctx->buf_gen++;
         if (e != hipSuccess) return check_hip(ctx, e, "hipMalloc(bins)");
-        ctx->bins_bytes = want_bytes;
+        bytes = want_bytes;
     }
-    char *b = (char *)ctx->bins;
+    char *b = (char *)buf;
     int rc = bins_upload_map(ctx, b, L, map, s);
     if (rc) return rc;
     bins_bind(b, L, map, st, job_shared);
@@ -430,6 +433,7 @@ int spx_ctx_create(int device, spx_ctx **out) {
     ctx->tuning.batch_blocks = 0;
     ctx->tuning.chunk_paths = 16384;
     ctx->tuning.chunk_photons = 8192;
+    ctx->tuning.overlap_bins = 0;
     *out = ctx;
     return SPX_OK;
 }
@@ -440,7 +444,7 @@ int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
         t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30 ||
         t->tile_wlog2 < kTileLog2Min || t->tile_wlog2 > kTileLog2Max || t->batch_blocks > 65536 ||
         t->chunk_paths < 64 || t->chunk_paths > (1u << 22) || t->chunk_photons < 64 ||
-        t->chunk_photons > (1u << 24))
+        t->chunk_photons > (1u << 24) || t->overlap_bins > 1)
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
     return SPX_OK;
@@ -464,7 +468,10 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->replica) hipFree(ctx->replica);
-    if (ctx->bins) hipFree(ctx->bins);
+    for (int k = 0; k < 2; ++k)
+        if (ctx->bins[k]) hipFree(ctx->bins[k]);
+    if (ctx->aux) hipStreamDestroy(ctx->aux);
+    for (hipEvent_t ev : ctx->sync_ev) hipEventDestroy(ev);
     if (ctx->stage) hipHostFree(ctx->stage);
     if (ctx->stage_done) hipEventDestroy(ctx->stage_done);
     if (ctx->batch_dev) hipFree(ctx->batch_dev);

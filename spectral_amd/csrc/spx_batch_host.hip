@@ -32,6 +32,7 @@ namespace {
 struct Schedule {
     uint32_t splat_mode = 2, blocks = 0, n_rounds = 0, n_jobs = 0;
     bool binned = false;
+    bool overlap = false;  // double-buffered photon rounds (bin passes on a second stream)
     std::vector<BatchJob> table;
     std::vector<BatchItem> ordered;
     std::vector<BatchQueues> qh;   // per round; round 0 = camera items
@@ -194,6 +195,7 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         }
         S.n_rounds = r + 1;
         S.bin_want = (size_t)(most / kBinFill);
+        S.overlap = T.overlap_bins != 0 && S.n_rounds > 2;
     }
 
     // Film replicas for the photon jobs (atomic deposit modes only).
@@ -305,30 +307,86 @@ hipError_t timed(spx_ctx *ctx, bool timing, uint32_t kind, hipStream_t s, F laun
     return e;
 }
 
+// Deposit-round buffers of one launch: bst[1] / shared[1] are used only by
+// overlapped schedules; `ev` holds 2 ordering events per photon round and
+// `s2` is the second stream.
+struct RoundBufs {
+    BinState bst[2];
+    const uint32_t *shared[2] = {nullptr, nullptr};
+    hipStream_t s2 = nullptr;
+    hipEvent_t *ev = nullptr;
+};
+
+// Photon rounds of a schedule (non-empty rounds after the camera round).
+uint32_t photon_rounds(const Schedule &S) {
+    uint32_t n = 0;
+    for (uint32_t r = 1; r < S.n_rounds; ++r) n += round_items(S, r) ? 1 : 0;
+    return n;
+}
+
 // The kernel sequence of a schedule whose image is at `d`: claim counters
-// reset, replica clear, then per round the bin reset, the trace kernel and
-// the bin passes, and finally the replica sums.  Only kernel launches (no
-// memset or copy), so the sequence can be captured into a graph.
-hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const BinState &bst, const uint32_t *job_shared,
-                   float *replica, hipStream_t s, bool timing) {
+// reset, replica clear, the camera round, then per photon round the bin
+// reset, the trace kernel and the bin passes, and finally the replica sums.
+// Overlapped schedules alternate two record buffers: round k's bin passes
+// run on s2 while round k+1 traces on s; round k+2 reuses k's buffer only
+// after k's passes end, and the passes stay in round order on s2 (they add
+// into the same films).  Only kernel launches and event ordering (no memset
+// or copy), so the sequence can be captured into a graph.
+hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B, float *replica, hipStream_t s,
+                   bool timing) {
     hipError_t e = launch_queues_reset((BatchQueues *)d, S.n_rounds, S.qstride, s);
     if (e == hipSuccess && !S.reps.empty()) e = launch_fill(replica, 0.0f, S.rep_total, s);
     const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
+    auto trace = [&](uint32_t r, const BinState &bst) {
+        const bool camera = r == 0;
+        return timed(ctx, timing, camera ? SPX_KT_CAMERA : SPX_KT_PHOTON, s, [&] {
+            return launch_batch(camera, (const BatchJob *)(d + S.off_jobs), (const BatchItem *)(d + S.off_items),
+                                (BatchQueues *)(d + r * S.qstride), S.blocks, S.splat_mode, bst, s);
+        });
+    };
+    uint32_t k = 0;  // photon rounds issued
     for (uint32_t r = 0; r < S.n_rounds && e == hipSuccess; ++r) {
         if (round_items(S, r) == 0) continue;
-        const bool camera = r == 0;
-        if (S.binned && !camera) e = launch_bins_reset(bst, s);
+        if (r == 0) {
+            e = trace(0, B.bst[0]);
+            continue;
+        }
+        if (!S.overlap) {
+            if (S.binned) e = launch_bins_reset(B.bst[0], s);
+            if (e == hipSuccess) e = trace(r, B.bst[0]);
+            if (e == hipSuccess && S.binned)
+                e = timed(ctx, timing, SPX_KT_BINS, s,
+                          [&] { return launch_bins_resolve(B.bst[0], B.shared[0], acc_blocks, s); });
+            continue;
+        }
+        const uint32_t buf = k & 1u;
+        hipEvent_t traced = B.ev[2 * k], resolved = B.ev[2 * k + 1];
+        if (k >= 2) e = hipStreamWaitEvent(s, B.ev[2 * (k - 2) + 1], 0);  // buffer free again
+        if (e == hipSuccess) e = launch_bins_reset(B.bst[buf], s);
+        if (e == hipSuccess) e = trace(r, B.bst[buf]);
+        if (e == hipSuccess) e = hipEventRecord(traced, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(B.s2, traced, 0);
         if (e == hipSuccess)
-            e = timed(ctx, timing, camera ? SPX_KT_CAMERA : SPX_KT_PHOTON, s, [&] {
-                return launch_batch(camera, (const BatchJob *)(d + S.off_jobs), (const BatchItem *)(d + S.off_items),
-                                    (BatchQueues *)(d + r * S.qstride), S.blocks, S.splat_mode, bst, s);
-            });
-        if (e == hipSuccess && S.binned && !camera)
-            e = timed(ctx, timing, SPX_KT_BINS, s, [&] { return launch_bins_resolve(bst, job_shared, acc_blocks, s); });
+            e = timed(ctx, timing, SPX_KT_BINS, B.s2,
+                      [&] { return launch_bins_resolve(B.bst[buf], B.shared[buf], acc_blocks, B.s2); });
+        if (e == hipSuccess) e = hipEventRecord(resolved, B.s2);
+        ++k;
     }
+    if (e == hipSuccess && S.overlap && k) e = hipStreamWaitEvent(s, B.ev[2 * (k - 1) + 1], 0);  // join
     if (e == hipSuccess && !S.reps.empty())
         e = launch_reduce_replicas((const ReplicaJob *)(d + S.off_reps), (uint32_t)S.reps.size(), S.rep_blocks, s);
     return e;
+}
+
+// `n` ordering events (no timing) in `pool`, created as needed.
+hipError_t ensure_events(std::vector<hipEvent_t> &pool, size_t n) {
+    while (pool.size() < n) {
+        hipEvent_t ev;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        pool.push_back(ev);
+    }
+    return hipSuccess;
 }
 
 int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
@@ -367,7 +425,8 @@ int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
 // Identity of a schedule's layout: everything the captured kernels and the
 // buffer sizes depend on, apart from the uploaded per-job values.
 bool same_layout(const Schedule &a, const Schedule &b) {
-    if (a.n_jobs != b.n_jobs || a.n_rounds != b.n_rounds || a.binned != b.binned || a.splat_mode != b.splat_mode ||
+    if (a.n_jobs != b.n_jobs || a.n_rounds != b.n_rounds || a.binned != b.binned || a.overlap != b.overlap ||
+        a.splat_mode != b.splat_mode ||
         a.blocks != b.blocks || a.bytes != b.bytes || a.rep_total != b.rep_total || a.bin_want != b.bin_want ||
         a.ordered.size() != b.ordered.size() || a.reps.size() != b.reps.size())
         return false;
@@ -403,12 +462,13 @@ struct spx_batch_plan {
     spx::Schedule S;
     char *dev = nullptr;   // schedule image
     char *host = nullptr;  // pinned copy of the image
-    char *bins = nullptr;  // deposit round buffers
+    char *bins[2] = {nullptr, nullptr};  // deposit round buffers (second: overlapped rounds)
     spx::BinLayout bl;
-    spx::BinState bst;
-    const uint32_t *job_shared = nullptr;
+    spx::RoundBufs rb;
     float *replica = nullptr;
-    hipStream_t cap = nullptr;  // capture stream
+    hipStream_t cap = nullptr;   // capture stream
+    hipStream_t cap2 = nullptr;  // second capture stream (overlapped bin passes)
+    std::vector<hipEvent_t> ev;  // capture-time ordering events
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t uploaded = nullptr;  // after the last image upload from `host`
@@ -423,17 +483,30 @@ static void plan_free(spx_batch_plan *p) {
     if (p->exec) hipGraphExecDestroy(p->exec);
     if (p->graph) hipGraphDestroy(p->graph);
     if (p->cap) hipStreamDestroy(p->cap);
+    if (p->cap2) hipStreamDestroy(p->cap2);
+    for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
     if (p->uploaded) hipEventDestroy(p->uploaded);
     if (!p->shared) {
         if (p->dev) hipFree(p->dev);
-        if (p->bins) hipFree(p->bins);
+        for (char *b : p->bins)
+            if (b) hipFree(b);
         if (p->replica) hipFree(p->replica);
         if (p->host) hipHostFree(p->host);
     }
     delete p;
 }
 
-// Uploads the plan's image and job map on `s` (stream-ordered).
+// Stream-ordered upload of the job map into every deposit buffer of a plan.
+static int plan_upload_maps(spx_batch_plan *p, hipStream_t s) {
+    if (!p->S.binned) return SPX_OK;
+    for (int k = 0; k < (p->S.overlap ? 2 : 1); ++k) {
+        int rc = bins_upload_map(p->ctx, p->bins[k], p->bl, p->S.map, s);
+        if (rc) return rc;
+    }
+    return SPX_OK;
+}
+
+// Uploads the plan's image and job maps on `s` (stream-ordered).
 static int plan_upload(spx_batch_plan *p, hipStream_t s) {
     spx_ctx *ctx = p->ctx;
     if (p->shared) {
@@ -448,8 +521,7 @@ static int plan_upload(spx_batch_plan *p, hipStream_t s) {
         if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
         if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_plan upload");
         ctx->batch_pending = true;
-        if (p->S.binned) return bins_upload_map(ctx, p->bins, p->bl, p->S.map, s);
-        return SPX_OK;
+        return plan_upload_maps(p, s);
     }
     if (p->upload_pending) {
         hipError_t e = hipEventSynchronize(p->uploaded);
@@ -461,25 +533,66 @@ static int plan_upload(spx_batch_plan *p, hipStream_t s) {
     if (e == hipSuccess) e = hipEventRecord(p->uploaded, s);
     if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_plan upload");
     p->upload_pending = true;
-    if (p->S.binned) return bins_upload_map(ctx, p->bins, p->bl, p->S.map, s);
-    return SPX_OK;
+    return plan_upload_maps(p, s);
+}
+
+// Binds the plan's deposit buffers (already sized for p->bl) into p->rb.
+static void plan_bind(spx_batch_plan *p) {
+    memset(&p->rb.bst, 0, sizeof p->rb.bst);
+    if (!p->S.binned) return;
+    for (int k = 0; k < (p->S.overlap ? 2 : 1); ++k) {
+        bins_bind(p->bins[k], p->bl, p->S.map, p->rb.bst[k], &p->rb.shared[k]);
+        p->rb.bst[k].atomic_film = p->S.overlap ? 1u : 0u;
+    }
 }
 
 // Captures the plan's kernel sequence into a graph (nothing to capture for
-// a table of empty jobs).
+// a table of empty jobs).  Overlapped schedules fork onto a second capture
+// stream and join back, so the graph holds both branches.
 static int plan_capture(spx_batch_plan *p) {
     if (p->S.ordered.empty()) return SPX_OK;
     spx_ctx *ctx = p->ctx;
     hipError_t e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
+    if (e == hipSuccess && p->S.overlap) {
+        e = hipStreamCreateWithFlags(&p->cap2, hipStreamNonBlocking);
+        if (e == hipSuccess) e = ensure_events(p->ev, 2 * (size_t)photon_rounds(p->S));
+        p->rb.s2 = p->cap2;
+        p->rb.ev = p->ev.data();
+    }
     if (e == hipSuccess) e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
-        hipError_t el = enqueue(ctx, p->S, p->dev, p->bst, p->job_shared, p->replica, p->cap, false);
+        hipError_t el = enqueue(ctx, p->S, p->dev, p->rb, p->replica, p->cap, false);
         e = hipStreamEndCapture(p->cap, &p->graph);
         if (el != hipSuccess) e = el;
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     if (e == hipSuccess) e = hipGraphGetNodes(p->graph, nullptr, &p->nodes);
     return check_hip(ctx, e, "spx_batch_plan capture");
+}
+
+// The context's second stream and ordering events for an overlapped run.
+static int ctx_overlap_streams(spx_ctx *ctx, const Schedule &S, RoundBufs &B) {
+    if (!S.overlap) return SPX_OK;
+    hipError_t e = hipSuccess;
+    if (!ctx->aux) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
+    if (e == hipSuccess) e = ensure_events(ctx->sync_ev, 2 * (size_t)photon_rounds(S));
+    if (e != hipSuccess) return check_hip(ctx, e, "overlap streams");
+    B.s2 = ctx->aux;
+    B.ev = ctx->sync_ev.data();
+    return SPX_OK;
+}
+
+// Sizes the context's deposit buffers for S (growing them may move them)
+// and binds them into B.
+static int ctx_bins(spx_ctx *ctx, const Schedule &S, hipStream_t s, RoundBufs &B) {
+    memset(&B.bst, 0, sizeof B.bst);
+    if (!S.binned) return SPX_OK;
+    for (int k = 0; k < (S.overlap ? 2 : 1); ++k) {
+        int rc = prepare_bins(ctx, S.map, S.bin_want, s, B.bst[k], &B.shared[k], k);
+        if (rc) return rc;
+        B.bst[k].atomic_film = S.overlap ? 1u : 0u;
+    }
+    return SPX_OK;
 }
 
 int spx::batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, hipStream_t s,
@@ -492,7 +605,8 @@ int spx::batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_
     int rc = plan_schedule(ctx, jobs, n_jobs, nullptr, p->S);
     Schedule &S = p->S;
     // grow the context's buffers first (this may move them), then bind
-    if (rc == SPX_OK && S.binned) rc = prepare_bins(ctx, S.map, S.bin_want, s, p->bst, &p->job_shared);
+    RoundBufs tmp;
+    if (rc == SPX_OK) rc = ctx_bins(ctx, S, s, tmp);
     if (rc == SPX_OK && !S.reps.empty()) rc = ensure_replicas(ctx, S.rep_total, s);
     if (rc == SPX_OK) rc = batch_stage(ctx, S.bytes, s);
     if (rc != SPX_OK) {
@@ -501,11 +615,10 @@ int spx::batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_
     }
     if (S.binned) {
         p->bl = bins_layout(S.map, S.bin_want, ctx->tuning.bin_log2);
-        p->bins = (char *)ctx->bins;
-        bins_bind(p->bins, p->bl, S.map, p->bst, &p->job_shared);  // the map was uploaded by prepare_bins
-    } else {
-        memset(&p->bst, 0, sizeof p->bst);
+        p->bins[0] = (char *)ctx->bins[0];
+        p->bins[1] = (char *)ctx->bins[1];
     }
+    plan_bind(p);  // the maps were uploaded by prepare_bins
     if (!S.reps.empty()) {
         p->replica = ctx->replica;
         bind_replicas(S, p->replica);
@@ -555,10 +668,8 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     int rc = plan_schedule(ctx, jobs, n_jobs, cfg, S);
     if (rc) return rc;
     if (S.ordered.empty()) return SPX_OK;
-    BinState bst;
-    memset(&bst, 0, sizeof bst);
-    const uint32_t *job_shared = nullptr;
-    if (S.binned && (rc = prepare_bins(ctx, S.map, S.bin_want, s, bst, &job_shared))) return rc;
+    RoundBufs B;
+    if ((rc = ctx_bins(ctx, S, s, B)) || (rc = ctx_overlap_streams(ctx, S, B))) return rc;
     if (!S.reps.empty()) {
         if ((rc = ensure_replicas(ctx, S.rep_total, s))) return rc;
         bind_replicas(S, ctx->replica);
@@ -570,7 +681,7 @@ int spx_batch_run(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     if (e == hipSuccess) e = hipEventRecord(ctx->batch_done, s);
     if (e != hipSuccess) return check_hip(ctx, e, "spx_batch_run upload");
     ctx->batch_pending = true;
-    e = enqueue(ctx, S, d, bst, job_shared, ctx->replica, s, true);
+    e = enqueue(ctx, S, d, B, ctx->replica, s, true);
     return check_hip(ctx, e, "spx_batch_run launch");
 }
 
@@ -596,8 +707,7 @@ int spx_batch_plan_create(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, co
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming);
     if (e == hipSuccess && S.binned) {
         p->bl = bins_layout(S.map, S.bin_want, ctx->tuning.bin_log2);
-        e = hipMalloc(&p->bins, p->bl.total);
-        if (e == hipSuccess) bins_bind(p->bins, p->bl, S.map, p->bst, &p->job_shared);
+        for (int k = 0; k < (S.overlap ? 2 : 1) && e == hipSuccess; ++k) e = hipMalloc(&p->bins[k], p->bl.total);
     }
     if (e == hipSuccess && !S.reps.empty()) {
         e = hipMalloc(&p->replica, S.rep_total * sizeof(float));
@@ -608,7 +718,7 @@ int spx_batch_plan_create(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, co
         plan_free(p);
         return rc;
     }
-    if (!S.binned) memset(&p->bst, 0, sizeof p->bst);
+    plan_bind(p);
     if ((rc = plan_upload(p, s))) {
         plan_free(p);
         return rc;
@@ -740,6 +850,7 @@ int spx_autotune(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, uint32_t sw
         {&spx_tuning::batch_blocks, {cus * 2, cus * 4, cus * 8}, false},
         {&spx_tuning::tile_wlog2, {3, 4, 5, 6, 7}, true},
         {&spx_tuning::bin_log2, {27, 28, 29, 30}, true},
+        {&spx_tuning::overlap_bins, {0, 1}, true},
     };
     spx_tuning best_t = start;
     if (best_t.batch_blocks == 0) best_t.batch_blocks = cus * 4;

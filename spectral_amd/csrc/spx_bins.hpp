@@ -86,6 +86,10 @@ struct BinState {
     uint32_t pool_cap;     // records per pool (multiple of kBinChunk)
     uint32_t n_strips;
     uint32_t tile_wlog2;   // tiles are 2^tile_wlog2 wide and 2^(10 - tile_wlog2) tall
+    // Film writes of the accumulation are atomic adds: set while the next
+    // round's trace kernel runs concurrently (its pool-overflow fallback adds
+    // into the same films).
+    uint32_t atomic_film;
 };
 
 __device__ inline uint32_t xcc_id() {

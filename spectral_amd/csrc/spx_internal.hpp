@@ -61,8 +61,10 @@ struct spx_ctx {
     float *replica = nullptr;     // grow-only film replica buffer
     size_t replica_floats = 0;
     spx_tuning tuning;
-    void *bins = nullptr;         // grow-only binned-deposit buffers
-    size_t bins_bytes = 0;
+    void *bins[2] = {nullptr, nullptr};  // grow-only binned-deposit buffers (2nd: overlapped rounds)
+    size_t bins_bytes[2] = {0, 0};
+    hipStream_t aux = nullptr;           // second stream of overlapped bin passes
+    std::vector<hipEvent_t> sync_ev;     // cross-stream ordering events (no timing)
     void *stage = nullptr;        // pinned staging for small stream-ordered uploads
     size_t stage_bytes = 0;
     hipEvent_t stage_done = nullptr;
@@ -171,7 +173,7 @@ void bins_bind(char *base, const BinLayout &L, const BinJobMap &map, BinState &s
 int bins_upload_map(spx_ctx *ctx, char *base, const BinLayout &L, const BinJobMap &map, hipStream_t s);
 // The above on the context's grow-only buffer.
 int prepare_bins(spx_ctx *ctx, const BinJobMap &map, size_t want, hipStream_t s, BinState &st,
-                 const uint32_t **job_shared);
+                 const uint32_t **job_shared, int slot = 0);
 // Share of the record capacity a round is planned to fill.
 constexpr double kBinFill = 0.85;
 // Expected film deposits per photon: the prism-and-floor volume renders

@@ -43,6 +43,30 @@ def test_batch_equals_alone(ctx, splat_mode, bin_log2):
         br.close()
 
 
+@pytest.mark.parametrize("splat_mode", [2, 3])
+def test_overlapped_rounds_equal_sequential(ctx, splat_mode):
+    # Double-buffered deposit rounds (bin passes of round k on a second
+    # stream while round k+1 traces) give the films of sequential rounds;
+    # bin_log2 = 15 forces many rounds and pool overflow into film atomics.
+    specs = _small_mix()
+    br = jobs.BatchRunner(ctx, specs)
+    old = ctx.tuning()
+    try:
+        films = []
+        for overlap in (0, 1):
+            ctx.set_tuning(splat_mode=splat_mode, bin_log2=15, overlap_bins=overlap)
+            br.clear()
+            br.run_batched()
+            films.append([f.image() for f in br.films])
+        for j, (a, b) in enumerate(zip(*films)):
+            assert a.sum() > 0
+            err = rel_l1_per_channel(b, a)
+            assert (err < ALONE_TOL).all(), f"job {j}: {err}"
+    finally:
+        ctx.set_tuning(**old.as_dict())
+        br.close()
+
+
 @pytest.mark.parametrize("splat_mode", [1, 2, 3])
 def test_batch_matches_oracle(ctx, oracle, splat_mode):
     specs = _small_mix(8)

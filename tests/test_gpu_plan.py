@@ -23,12 +23,18 @@ def _images(br):
     return [f.image().copy() for f in br.films]
 
 
-@pytest.mark.parametrize("splat_mode", [1, 2, 3])
-def test_plan_replays_equal_batch_run(ctx, splat_mode):
+@pytest.mark.parametrize("splat_mode,overlap", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
+def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap):
+    # overlap = 1: the graph forks the bin passes onto a second branch
+    # (double-buffered rounds; bin_log2 = 21 makes several rounds without
+    # overflowing the record pools, whose fallback atomics would reorder sums)
     specs = _small_mix()
     br = jobs.BatchRunner(ctx, specs)
     cfg = A.BatchCfg(splat_mode, 0, 4096, 2048)
     plan = None
+    old = ctx.tuning()
+    if overlap:
+        ctx.set_tuning(overlap_bins=1, bin_log2=21)
     try:
         br.clear()
         br.run_batched(cfg=cfg)
@@ -36,7 +42,7 @@ def test_plan_replays_equal_batch_run(ctx, splat_mode):
         plan = br.plan(cfg=cfg)
         info = plan.info()
         print("plan", info)
-        assert info["rounds"] >= 2 and info["graph_nodes"] >= 3
+        assert info["rounds"] >= (4 if overlap else 2) and info["graph_nodes"] >= 3
         # several replays, with eager work and synchronisation between them
         for rep in range(3):
             br.clear()
@@ -59,6 +65,7 @@ def test_plan_replays_equal_batch_run(ctx, splat_mode):
     finally:
         if plan is not None:
             plan.close()
+        ctx.set_tuning(**old.as_dict())
         br.close()
 
 
