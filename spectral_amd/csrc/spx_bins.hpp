@@ -152,7 +152,16 @@ struct BinSink {
     uint32_t tiles_x;    // tiles across the film
     uint32_t tile_wlog2; // tile width log2
 
-    __device__ inline void add(uint32_t idx, f3 v) {
+    __device__ inline void add(uint32_t idx, f3 v) { append(tile_key(idx, width, tiles_x, tile_wlog2), idx, v); }
+    // The same deposit when its pixel (px, py) is at hand: no division.
+    __device__ inline void add_xy(uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+        const uint32_t th = kStripLog2 - tile_wlog2;
+        const uint32_t tile = (py >> th) * tiles_x + (px >> tile_wlog2);
+        const uint32_t local = ((py & ((1u << th) - 1u)) << tile_wlog2) | (px & ((1u << tile_wlog2) - 1u));
+        append((tile << kStripLog2) | local, idx, v);
+    }
+
+    __device__ inline void append(uint32_t key, uint32_t idx, f3 v) {
         uint64_t m = __ballot(1);
         uint32_t lane = __lane_id();
         uint32_t cnt = (uint32_t)__popcll(m);
@@ -182,7 +191,7 @@ struct BinSink {
             return;
         }
         BinRec r;
-        r.key = key_base + tile_key(idx, width, tiles_x, tile_wlog2);
+        r.key = key_base + key;
         r.x = v.x;
         r.y = v.y;
         r.z = v.z;

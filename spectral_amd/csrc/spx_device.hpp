@@ -346,12 +346,16 @@ __device__ inline bool project(const Proj &P, f3 p, float &s, float &t) {
     return false;
 }
 
-// Nearest-pixel index, volume.rs:134-135.
-__device__ inline uint32_t pix_index(float s, float t, uint32_t w, uint32_t h) {
-    uint32_t px = (uint32_t)(s * (float)w);
-    uint32_t py = (uint32_t)((1.0f - t) * (float)h);
+// Nearest pixel, volume.rs:134-135.
+__device__ inline void pix_xy(float s, float t, uint32_t w, uint32_t h, uint32_t &px, uint32_t &py) {
+    px = (uint32_t)(s * (float)w);
+    py = (uint32_t)((1.0f - t) * (float)h);
     px = px > w - 1 ? w - 1 : px;
     py = py > h - 1 ? h - 1 : py;
+}
+__device__ inline uint32_t pix_index(float s, float t, uint32_t w, uint32_t h) {
+    uint32_t px, py;
+    pix_xy(s, t, w, h, px, py);
     return py * w + px;
 }
 

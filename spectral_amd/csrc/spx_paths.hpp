@@ -134,6 +134,13 @@ __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, ui
     st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
 }
 
+// A deposit at film pixel (px, py) = index idx; sinks keyed by film tile
+// overload it to use the pixel directly.
+template <class Sink>
+__device__ inline void deposit_xy(Sink &sink, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+    sink.add(idx, v);
+}
+
 // Hooks a deposit sink may take: a new photon starts, the photon scattered
 // off a material of `kind`.  Sinks that track photons overload these.
 template <class Sink>
@@ -192,10 +199,10 @@ __device__ inline void screen_splat(const PhotonParams &P, const float4 *tbl, fl
     float w00 = (1.0f - dx) * (1.0f - dy), w10 = dx * (1.0f - dy);
     float w01 = (1.0f - dx) * dy, w11 = dx * dy;
     f3 cp = c * power;
-    sink.add(y0 * P.scr_w + x0, cp * w00);
-    sink.add(y0 * P.scr_w + x1, cp * w10);
-    sink.add(y1 * P.scr_w + x0, cp * w01);
-    sink.add(y1 * P.scr_w + x1, cp * w11);
+    deposit_xy(sink, x0, y0, y0 * P.scr_w + x0, cp * w00);
+    deposit_xy(sink, x1, y0, y0 * P.scr_w + x1, cp * w10);
+    deposit_xy(sink, x0, y1, y1 * P.scr_w + x0, cp * w01);
+    deposit_xy(sink, x1, y1, y1 * P.scr_w + x1, cp * w11);
 }
 
 // One lighttrace bounce (trace_photon body, lighttrace.rs:133-151).
@@ -247,11 +254,13 @@ __device__ inline void point_deposits(const PhotonParams &P, const PhotonState &
         f3 p = st.ro + st.rd * dist;
         float s, t;
         if (!project(P.proj, p, s, t)) continue;
-        uint32_t idx = pix_index(s, t, P.w, P.h);
+        uint32_t px, py;
+        pix_xy(s, t, P.w, P.h, px, py);
+        const uint32_t idx = py * P.w + px;
         if (LEGACY) {
             const float inv4pi = 1.0f / 12.566371f;
             float contrib = st.power * P.vp.sigma_s * seg_len / 4.0f * inv4pi;
-            sink.add(idx, cmf * contrib);
+            deposit_xy(sink, px, py, idx, cmf * contrib);
         } else {
             f3 to = cam_o - p;
             float d = fmaxf(length(to), 1e-3f);
@@ -259,7 +268,7 @@ __device__ inline void point_deposits(const PhotonParams &P, const PhotonState &
             float phase = ISO ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
             float trans = expf(-P.vp.sigma_t * d);
             float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
-            sink.add(idx, cmf * contrib);
+            deposit_xy(sink, px, py, idx, cmf * contrib);
         }
     }
 }

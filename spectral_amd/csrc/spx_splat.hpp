@@ -127,6 +127,9 @@ struct HashBinSink {
     __device__ inline void add(uint32_t idx, f3 v) {
         if (spread || !hash.try_add(idx, v)) bins.add(idx, v);
     }
+    __device__ inline void add_xy(uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+        if (spread || !hash.try_add(idx, v)) bins.add_xy(px, py, idx, v);
+    }
     // Whole block must call.
     __device__ inline void flush() {
         hash.drain([&](uint32_t key, f3 v) { bins.add(key, v); });
@@ -147,6 +150,16 @@ __device__ inline void sink_photon_begin(HashBinSink<SLOTS> &s) {
 template <int SLOTS>
 __device__ inline void sink_scattered(HashBinSink<SLOTS> &s, uint32_t kind) {
     if (kind == SPX_MAT_LAMBERTIAN) s.spread = true;
+}
+
+// A deposit at film pixel (px, py) = index idx: sinks that key records by
+// film tile take the pixel itself.
+__device__ inline void deposit_xy(BinSink &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+    s.add_xy(px, py, idx, v);
+}
+template <int SLOTS>
+__device__ inline void deposit_xy(HashBinSink<SLOTS> &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+    s.add_xy(px, py, idx, v);
 }
 
 }  // namespace spx
