@@ -176,6 +176,9 @@ typedef struct {
     uint32_t overlap_bins;   /* batched engine, binned deposits: 1 = double-buffered rounds, the bin
                                 passes of round r on a second stream while round r+1 traces
                                 (two record buffers); default 0 */
+    uint32_t camera_blocks;  /* batched engine: 0 = the camera round runs first, on the whole grid;
+                                N > 0 = it runs on its own stream with N blocks, concurrently with
+                                the photon rounds; default 0 */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */

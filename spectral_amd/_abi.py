@@ -91,7 +91,8 @@ class Tuning(C.Structure):
     _fields_ = [("max_replicas", C.c_uint32), ("replica_mib", C.c_uint32),
                 ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32), ("bin_log2", C.c_uint32),
                 ("tile_wlog2", C.c_uint32), ("batch_blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
-                ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32)]
+                ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32),
+                ("camera_blocks", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
@@ -99,7 +100,8 @@ class Tuning(C.Structure):
 
 class BatchCfg(C.Structure):
     _fields_ = [("splat_mode", C.c_uint32), ("blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
-                ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32)]
+                ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32),
+                ("camera_blocks", C.c_uint32)]
 
 
 class Request(C.Structure):

@@ -434,6 +434,7 @@ int spx_ctx_create(int device, spx_ctx **out) {
     ctx->tuning.chunk_paths = 16384;
     ctx->tuning.chunk_photons = 8192;
     ctx->tuning.overlap_bins = 0;
+    ctx->tuning.camera_blocks = 0;
     *out = ctx;
     return SPX_OK;
 }
@@ -444,7 +445,7 @@ int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
         t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30 ||
         t->tile_wlog2 < kTileLog2Min || t->tile_wlog2 > kTileLog2Max || t->batch_blocks > 65536 ||
         t->chunk_paths < 64 || t->chunk_paths > (1u << 22) || t->chunk_photons < 64 ||
-        t->chunk_photons > (1u << 24) || t->overlap_bins > 1)
+        t->chunk_photons > (1u << 24) || t->overlap_bins > 1 || t->camera_blocks > 65536)
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
     return SPX_OK;
@@ -471,6 +472,7 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     for (int k = 0; k < 2; ++k)
         if (ctx->bins[k]) hipFree(ctx->bins[k]);
     if (ctx->aux) hipStreamDestroy(ctx->aux);
+    if (ctx->aux_cam) hipStreamDestroy(ctx->aux_cam);
     for (hipEvent_t ev : ctx->sync_ev) hipEventDestroy(ev);
     if (ctx->stage) hipHostFree(ctx->stage);
     if (ctx->stage_done) hipEventDestroy(ctx->stage_done);

@@ -33,6 +33,7 @@ struct Schedule {
     uint32_t splat_mode = 2, blocks = 0, n_rounds = 0, n_jobs = 0;
     bool binned = false;
     bool overlap = false;  // double-buffered photon rounds (bin passes on a second stream)
+    uint32_t cam_blocks = 0;  // > 0: the camera round runs concurrently, on its own stream
     std::vector<BatchJob> table;
     std::vector<BatchItem> ordered;
     std::vector<BatchQueues> qh;   // per round; round 0 = camera items
@@ -250,6 +251,10 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
             S.qh[r].begin[q] = start;
             start += S.qh[r].size[q];
         }
+    uint32_t cam_items = 0, ph_items = 0;
+    for (uint32_t q = 0; q < kXcdQueues; ++q) cam_items += S.qh[0].size[q];
+    ph_items = (uint32_t)S.ordered.size() - cam_items;
+    S.cam_blocks = T.camera_blocks && cam_items && ph_items ? T.camera_blocks : 0;
     S.qstride = a256(sizeof(BatchQueues));
     S.off_items = S.qstride * S.n_rounds;
     S.off_jobs = a256(S.off_items + S.ordered.size() * sizeof(BatchItem));
@@ -314,7 +319,9 @@ struct RoundBufs {
     BinState bst[2];
     const uint32_t *shared[2] = {nullptr, nullptr};
     hipStream_t s2 = nullptr;
-    hipEvent_t *ev = nullptr;
+    hipEvent_t *ev = nullptr;   // overlapped rounds: 2 per photon round
+    hipStream_t s3 = nullptr;   // concurrent camera round
+    hipEvent_t *cev = nullptr;  // its fork and done events
 };
 
 // Photon rounds of a schedule (non-empty rounds after the camera round).
@@ -337,18 +344,25 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
     hipError_t e = launch_queues_reset((BatchQueues *)d, S.n_rounds, S.qstride, s);
     if (e == hipSuccess && !S.reps.empty()) e = launch_fill(replica, 0.0f, S.rep_total, s);
     const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
-    auto trace = [&](uint32_t r, const BinState &bst) {
+    auto trace_on = [&](uint32_t r, const BinState &bst, hipStream_t st, uint32_t blocks) {
         const bool camera = r == 0;
-        return timed(ctx, timing, camera ? SPX_KT_CAMERA : SPX_KT_PHOTON, s, [&] {
+        return timed(ctx, timing, camera ? SPX_KT_CAMERA : SPX_KT_PHOTON, st, [&] {
             return launch_batch(camera, (const BatchJob *)(d + S.off_jobs), (const BatchItem *)(d + S.off_items),
-                                (BatchQueues *)(d + r * S.qstride), S.blocks, S.splat_mode, bst, s);
+                                (BatchQueues *)(d + r * S.qstride), blocks, S.splat_mode, bst, st);
         });
     };
+    auto trace = [&](uint32_t r, const BinState &bst) { return trace_on(r, bst, s, S.blocks); };
+    if (e == hipSuccess && S.cam_blocks) {  // fork the camera round
+        e = hipEventRecord(B.cev[0], s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(B.s3, B.cev[0], 0);
+        if (e == hipSuccess) e = trace_on(0, B.bst[0], B.s3, S.cam_blocks);
+        if (e == hipSuccess) e = hipEventRecord(B.cev[1], B.s3);
+    }
     uint32_t k = 0;  // photon rounds issued
     for (uint32_t r = 0; r < S.n_rounds && e == hipSuccess; ++r) {
         if (round_items(S, r) == 0) continue;
         if (r == 0) {
-            e = trace(0, B.bst[0]);
+            if (!S.cam_blocks) e = trace(0, B.bst[0]);
             continue;
         }
         if (!S.overlap) {
@@ -373,6 +387,7 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
         ++k;
     }
     if (e == hipSuccess && S.overlap && k) e = hipStreamWaitEvent(s, B.ev[2 * (k - 1) + 1], 0);  // join
+    if (e == hipSuccess && S.cam_blocks) e = hipStreamWaitEvent(s, B.cev[1], 0);
     if (e == hipSuccess && !S.reps.empty())
         e = launch_reduce_replicas((const ReplicaJob *)(d + S.off_reps), (uint32_t)S.reps.size(), S.rep_blocks, s);
     return e;
@@ -426,7 +441,7 @@ int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
 // buffer sizes depend on, apart from the uploaded per-job values.
 bool same_layout(const Schedule &a, const Schedule &b) {
     if (a.n_jobs != b.n_jobs || a.n_rounds != b.n_rounds || a.binned != b.binned || a.overlap != b.overlap ||
-        a.splat_mode != b.splat_mode ||
+        a.splat_mode != b.splat_mode || a.cam_blocks != b.cam_blocks ||
         a.blocks != b.blocks || a.bytes != b.bytes || a.rep_total != b.rep_total || a.bin_want != b.bin_want ||
         a.ordered.size() != b.ordered.size() || a.reps.size() != b.reps.size())
         return false;
@@ -468,7 +483,9 @@ struct spx_batch_plan {
     float *replica = nullptr;
     hipStream_t cap = nullptr;   // capture stream
     hipStream_t cap2 = nullptr;  // second capture stream (overlapped bin passes)
+    hipStream_t cap3 = nullptr;  // third capture stream (concurrent camera round)
     std::vector<hipEvent_t> ev;  // capture-time ordering events
+    std::vector<hipEvent_t> cev;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t uploaded = nullptr;  // after the last image upload from `host`
@@ -484,7 +501,9 @@ static void plan_free(spx_batch_plan *p) {
     if (p->graph) hipGraphDestroy(p->graph);
     if (p->cap) hipStreamDestroy(p->cap);
     if (p->cap2) hipStreamDestroy(p->cap2);
+    if (p->cap3) hipStreamDestroy(p->cap3);
     for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
+    for (hipEvent_t ev : p->cev) hipEventDestroy(ev);
     if (p->uploaded) hipEventDestroy(p->uploaded);
     if (!p->shared) {
         if (p->dev) hipFree(p->dev);
@@ -559,6 +578,12 @@ static int plan_capture(spx_batch_plan *p) {
         p->rb.s2 = p->cap2;
         p->rb.ev = p->ev.data();
     }
+    if (e == hipSuccess && p->S.cam_blocks) {
+        e = hipStreamCreateWithFlags(&p->cap3, hipStreamNonBlocking);
+        if (e == hipSuccess) e = ensure_events(p->cev, 2);
+        p->rb.s3 = p->cap3;
+        p->rb.cev = p->cev.data();
+    }
     if (e == hipSuccess) e = hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         hipError_t el = enqueue(ctx, p->S, p->dev, p->rb, p->replica, p->cap, false);
@@ -570,15 +595,19 @@ static int plan_capture(spx_batch_plan *p) {
     return check_hip(ctx, e, "spx_batch_plan capture");
 }
 
-// The context's second stream and ordering events for an overlapped run.
+// The context's extra streams and ordering events for overlapped rounds and
+// a concurrent camera round (events: 2 for the camera, then 2 per round).
 static int ctx_overlap_streams(spx_ctx *ctx, const Schedule &S, RoundBufs &B) {
-    if (!S.overlap) return SPX_OK;
+    if (!S.overlap && !S.cam_blocks) return SPX_OK;
     hipError_t e = hipSuccess;
-    if (!ctx->aux) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
-    if (e == hipSuccess) e = ensure_events(ctx->sync_ev, 2 * (size_t)photon_rounds(S));
+    if (S.overlap && !ctx->aux) e = hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking);
+    if (e == hipSuccess && S.cam_blocks && !ctx->aux_cam) e = hipStreamCreateWithFlags(&ctx->aux_cam, hipStreamNonBlocking);
+    if (e == hipSuccess) e = ensure_events(ctx->sync_ev, 2 + 2 * (size_t)photon_rounds(S));
     if (e != hipSuccess) return check_hip(ctx, e, "overlap streams");
     B.s2 = ctx->aux;
-    B.ev = ctx->sync_ev.data();
+    B.s3 = ctx->aux_cam;
+    B.cev = ctx->sync_ev.data();
+    B.ev = ctx->sync_ev.data() + 2;
     return SPX_OK;
 }
 
@@ -851,6 +880,7 @@ int spx_autotune(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, uint32_t sw
         {&spx_tuning::tile_wlog2, {3, 4, 5, 6, 7}, true},
         {&spx_tuning::bin_log2, {27, 28, 29, 30}, true},
         {&spx_tuning::overlap_bins, {0, 1}, true},
+        {&spx_tuning::camera_blocks, {0, cus, cus * 2}, false},
     };
     spx_tuning best_t = start;
     if (best_t.batch_blocks == 0) best_t.batch_blocks = cus * 4;

@@ -64,6 +64,7 @@ struct spx_ctx {
     void *bins[2] = {nullptr, nullptr};  // grow-only binned-deposit buffers (2nd: overlapped rounds)
     size_t bins_bytes[2] = {0, 0};
     hipStream_t aux = nullptr;           // second stream of overlapped bin passes
+    hipStream_t aux_cam = nullptr;       // stream of a concurrent camera round
     std::vector<hipEvent_t> sync_ev;     // cross-stream ordering events (no timing)
     void *stage = nullptr;        // pinned staging for small stream-ordered uploads
     size_t stage_bytes = 0;

@@ -53,15 +53,17 @@ def test_overlapped_rounds_equal_sequential(ctx, splat_mode):
     old = ctx.tuning()
     try:
         films = []
-        for overlap in (0, 1):
-            ctx.set_tuning(splat_mode=splat_mode, bin_log2=15, overlap_bins=overlap)
+        # sequential; overlapped rounds; overlapped rounds + a concurrent camera round
+        for overlap, cam_blocks in ((0, 0), (1, 0), (1, 256)):
+            ctx.set_tuning(splat_mode=splat_mode, bin_log2=15, overlap_bins=overlap, camera_blocks=cam_blocks)
             br.clear()
             br.run_batched()
             films.append([f.image() for f in br.films])
-        for j, (a, b) in enumerate(zip(*films)):
-            assert a.sum() > 0
-            err = rel_l1_per_channel(b, a)
-            assert (err < ALONE_TOL).all(), f"job {j}: {err}"
+        for other in films[1:]:
+            for j, (a, b) in enumerate(zip(films[0], other)):
+                assert a.sum() > 0
+                err = rel_l1_per_channel(b, a)
+                assert (err < ALONE_TOL).all(), f"job {j}: {err}"
     finally:
         ctx.set_tuning(**old.as_dict())
         br.close()

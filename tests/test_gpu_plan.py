@@ -33,8 +33,8 @@ def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap):
     cfg = A.BatchCfg(splat_mode, 0, 4096, 2048)
     plan = None
     old = ctx.tuning()
-    if overlap:
-        ctx.set_tuning(overlap_bins=1, bin_log2=21)
+    if overlap:  # and the camera round on a third branch
+        ctx.set_tuning(overlap_bins=1, bin_log2=21, camera_blocks=256)
     try:
         br.clear()
         br.run_batched(cfg=cfg)
