@@ -109,8 +109,19 @@ struct TblPos {
     float frac;
     bool last;
 };
+// x / 5 correctly rounded, from one multiply and two FMAs: q = x * RN(1/5),
+// r = x - q * 5 exactly, q + r * RN(1/5).  Equal to the IEEE quotient for
+// every float x in [0, 350] (checked exhaustively, tests/test_div5.py);
+// tbl_pos clamps its quotient to [0, 70], so outside that range a last-bit
+// difference cannot reach the result either.
+__device__ inline float div5(float x) {
+    const float q = x * 0.2f;
+    const float r = fmaf(-q, 5.0f, x);
+    return fmaf(r, 0.2f, q);
+}
+
 __device__ inline TblPos tbl_pos(float nm) {
-    float f = (nm - 380.0f) / 5.0f;
+    float f = div5(nm - 380.0f);
     f = f < 0.0f ? 0.0f : f;
     f = f > (float)(kTblN - 1) ? (float)(kTblN - 1) : f;
     int i = (int)floorf(f);
