@@ -4,14 +4,14 @@ One step = the mixed stream of 512 uneven jobs (half rainbow camera renders,
 half prism-on-floor volume renders; spectral_amd.jobs.mixed_jobs, seed 0)
 traced by the batched engine, inputs resident in HBM: the table's schedule is
 captured once as a hipGraph (spx_batch_plan_create) and each step replays it.
-The launch shape is picked first by the autotuner (spx_autotune) on every
-fourth job of the stream.  value = (paths + photons) per second / 1e6 over
-exactly K timed steps.
+The launch shape is picked first by the autotuner (spx_autotune) on the same
+table.  value = (paths + photons) per second / 1e6 over exactly K timed steps.
 
 Also reported on the same line: the plain one-job-per-launch form on the same
-stream, per-kernel times of the batched step (HIP events around each kernel
-group of K eager steps on the launch stream) and the roofline of the dominant
-kernel, the per-configuration throughputs (DSOTM prism 2^20 photons at
+stream, the eager (ungraphed) step, per-kernel times of the batched step (HIP
+events around each kernel group of K eager steps whose rounds run in
+sequence, on the stream each group is launched on) and the roofline of the
+dominant kernel, the per-configuration throughputs (DSOTM prism 2^20 photons at
 320x180; rainbow 512^2 x 64 spp; full-size prism-on-floor 2^28 photons at
 1920x1080 under D65, A and dispersion off, plus its floor caustic), the
 served stream's p50/p99 job latency and the CPU oracle timed on this host.
@@ -303,16 +303,16 @@ def main() -> int:
     log(f"mixed stream: {len(specs)} jobs, {paths / 1e9:.3f} G paths, {photons / 1e9:.3f} G photons")
     br = jobs.BatchRunner(ctx, specs)
 
-    # launch shape: explicit, default, or autotuned on every fourth job
+    # launch shape: explicit, default, or autotuned on the step's own table
     tune_info = {"source": "default"}
     if args.tuning:
         ctx.set_tuning(**json.loads(args.tuning))
         tune_info = {"source": "given"}
     elif not args.no_tune:
         t0 = time.perf_counter()
-        _, tune_ms = br.autotune(stream, sweeps=1, subset=list(range(0, len(specs), 4)))
-        tune_info = {"source": "spx_autotune on every 4th job", "seconds": round(time.perf_counter() - t0, 1),
-                     "sample_ms": round(tune_ms, 3)}
+        _, tune_ms = br.autotune(stream, sweeps=1)
+        tune_info = {"source": "spx_autotune on the 512-job table", "seconds": round(time.perf_counter() - t0, 1),
+                     "tuned_ms": round(tune_ms, 3)}
     tuning = ctx.tuning().as_dict()
     log(f"tuning ({tune_info['source']}): {tuning}")
 
@@ -327,15 +327,23 @@ def main() -> int:
     value = total / ms_step / 1e3
     log(f"batched (graph replay): {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
     plan.close()
-    # per-kernel times: the same schedule run eagerly, each kernel group
-    # bracketed by HIP events on the launch stream
+    # the same schedule launched eagerly (no graph)
+    br.run_batched(stream)
+    ctx.sync(stream)
+    eager_ms, _ = timer.run(lambda: br.run_batched(stream), args.steps)
+    log(f"batched (eager): {eager_ms:.2f} ms/step")
+    # per-kernel times: the rounds run in sequence (bin passes not overlapped
+    # with the next round, camera round not concurrent), so each kernel
+    # group's HIP event pair on its stream brackets that group alone
+    ctx.set_tuning(**dict(tuning, overlap_bins=0, camera_blocks=0))
     br.run_batched(stream)
     ctx.sync(stream)
     ctx.kernel_timing(True)
-    eager_ms, _ = timer.run(lambda: br.run_batched(stream), args.steps)
+    seq_ms, _ = timer.run(lambda: br.run_batched(stream), args.steps)
     kt = ctx.kernel_times()
     ctx.kernel_timing(False)
-    log(f"batched (eager): {eager_ms:.2f} ms/step; kernel groups {kt}")
+    ctx.set_tuning(**tuning)
+    log(f"batched (eager, rounds in sequence): {seq_ms:.2f} ms/step; kernel groups {kt}")
     if args.no_plain:
         ms_plain = float("nan")
     else:
@@ -388,6 +396,7 @@ def main() -> int:
         "plain_value": None if args.no_plain else round(plain_value, 2),
         "batched_over_plain": None if args.no_plain else round(value / plain_value, 3),
         "eager_batched_ms_per_step": round(eager_ms, 3),
+        "sequential_batched_ms_per_step": round(seq_ms, 3),
         "tuning": dict(tuning, **tune_info),
         "roofline": roofline,
     }
