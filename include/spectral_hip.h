@@ -179,6 +179,10 @@ typedef struct {
     uint32_t camera_blocks;  /* batched engine: 0 = the camera round runs first, on the whole grid;
                                 N > 0 = it runs on its own stream with N blocks, concurrently with
                                 the photon rounds; default 0 */
+    uint32_t photon_threads; /* batched engine, deposit modes 1 and 3: threads per photon-kernel block,
+                                256 (LDS hash of 2048 pixels per block) or 1024 (one block per
+                                compute unit sharing a hash of 8192 pixels: more deposits meet
+                                in LDS); the grid keeps batch_blocks x 256 threads; default 256 */
 } spx_tuning;
 
 /* ---- context --------------------------------------------------------------- */

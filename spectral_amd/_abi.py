@@ -92,7 +92,7 @@ class Tuning(C.Structure):
                 ("splat_mode", C.c_uint32), ("chunk_log2", C.c_uint32), ("bin_log2", C.c_uint32),
                 ("tile_wlog2", C.c_uint32), ("batch_blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
                 ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32),
-                ("camera_blocks", C.c_uint32)]
+                ("camera_blocks", C.c_uint32), ("photon_threads", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

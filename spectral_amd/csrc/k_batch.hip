@@ -156,22 +156,27 @@ __global__ __launch_bounds__(256, 4) void k_batch_camera(const BatchJob *__restr
 // Photon items only (volume and lighttrace), deposits through sink SPLAT:
 // 0 = direct film atomics, 1 = LDS hash, 2 = binned records, 3 = LDS hash
 // draining to binned records.
-template <int SPLAT>
-__global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restrict__ jobs,
-                                                       const BatchItem *__restrict__ items, BatchQueues *q,
-                                                       BinState bins) {
+// THREADS: block size; the LDS hash holds 8 slots per thread (2048 at 256
+// threads; 8192 at 1024 threads, one block filling a compute unit, so all of
+// the unit's photons share one 128 KiB table).
+template <int SPLAT, int THREADS = 256>
+__global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__restrict__ jobs,
+                                                           const BatchItem *__restrict__ items, BatchQueues *q,
+                                                           BinState bins) {
+    constexpr int kSlots = THREADS / 256 * kBatchSlots;
+    constexpr int kWaves = THREADS / 64;
     __shared__ BatchBlock B;
-    __shared__ uint32_t sk[SPLAT == 1 || SPLAT == 3 ? kBatchSlots : 1];
-    __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kBatchSlots : 1];
+    __shared__ uint32_t sk[SPLAT == 1 || SPLAT == 3 ? kSlots : 1];
+    __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kSlots : 1];
     __shared__ uint32_t sfill;
-    __shared__ uint32_t wstate[2 * 4];  // binned deposits: {chunk, fill} per wave
-    LdsSplat<kBatchSlots, SPLAT == 3 ? kHashBinProbes : 8> hash{sk, sv, &sfill, nullptr};
+    __shared__ uint32_t wstate[2 * kWaves];  // binned deposits: {chunk, fill} per wave
+    LdsSplat<kSlots, SPLAT == 3 ? kHashBinProbes : 8> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
         B.scene = nullptr;
         B.tbl_src = nullptr;
     }
     if (SPLAT == 1 || SPLAT == 3) hash.init();
-    if (threadIdx.x < 8) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
+    if (threadIdx.x < 2 * kWaves) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
     const uint32_t pool = xcc_id() % kBinPools;
     const uint32_t home = blockIdx.x % kXcdQueues;
     uint32_t qi = 0;
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(256) void k_batch_photons(const BatchJob *__restric
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
         } else if constexpr (SPLAT == 3) {
             const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
-            HashBinSink<kBatchSlots> hb{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
+            HashBinSink<kSlots> hb{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
                                                       pool, P.key_base, P.film, wstate, fw,
                                                       tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hb);
@@ -224,11 +229,17 @@ hipError_t launch_queues_reset(BatchQueues *q, uint32_t n_rounds, size_t stride,
 }
 
 hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *items, BatchQueues *q,
-                        uint32_t blocks, uint32_t splat_mode, const BinState &bins, hipStream_t s) {
+                        uint32_t blocks, uint32_t splat_mode, uint32_t photon_threads, const BinState &bins,
+                        hipStream_t s) {
+    const bool wide = photon_threads == 1024 && (splat_mode == 1 || splat_mode == 3) && blocks >= 4;
     if (camera)
         hipLaunchKernelGGL(k_batch_camera, dim3(blocks), dim3(256), 0, s, jobs, items, q);
     else if (splat_mode == 0)
         hipLaunchKernelGGL(k_batch_photons<0>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+    else if (wide && splat_mode == 1)
+        hipLaunchKernelGGL((k_batch_photons<1, 1024>), dim3(blocks / 4), dim3(1024), 0, s, jobs, items, q, bins);
+    else if (wide)
+        hipLaunchKernelGGL((k_batch_photons<3, 1024>), dim3(blocks / 4), dim3(1024), 0, s, jobs, items, q, bins);
     else if (splat_mode == 1)
         hipLaunchKernelGGL(k_batch_photons<1>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
     else if (splat_mode == 2)

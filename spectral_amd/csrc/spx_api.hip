@@ -435,6 +435,7 @@ int spx_ctx_create(int device, spx_ctx **out) {
     ctx->tuning.chunk_photons = 8192;
     ctx->tuning.overlap_bins = 0;
     ctx->tuning.camera_blocks = 0;
+    ctx->tuning.photon_threads = 256;
     *out = ctx;
     return SPX_OK;
 }
@@ -445,7 +446,8 @@ int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
         t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30 ||
         t->tile_wlog2 < kTileLog2Min || t->tile_wlog2 > kTileLog2Max || t->batch_blocks > 65536 ||
         t->chunk_paths < 64 || t->chunk_paths > (1u << 22) || t->chunk_photons < 64 ||
-        t->chunk_photons > (1u << 24) || t->overlap_bins > 1 || t->camera_blocks > 65536)
+        t->chunk_photons > (1u << 24) || t->overlap_bins > 1 || t->camera_blocks > 65536 ||
+        (t->photon_threads != 256 && t->photon_threads != 1024))
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
     return SPX_OK;

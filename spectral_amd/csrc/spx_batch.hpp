@@ -29,8 +29,11 @@ struct BatchQueues {
 };
 
 // camera: the camera kernel (items of camera jobs only); else the photon
-// kernel for splat_mode (items of volume / lighttrace jobs only).
+// kernel for splat_mode (items of volume / lighttrace jobs only).  `blocks`
+// counts blocks of 256 threads; with photon_threads 1024 the hash modes run
+// blocks / 4 blocks of 1024 threads.
 hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *items, BatchQueues *q,
-                        uint32_t blocks, uint32_t splat_mode, const BinState &bins, hipStream_t s);
+                        uint32_t blocks, uint32_t splat_mode, uint32_t photon_threads, const BinState &bins,
+                        hipStream_t s);
 
 }  // namespace spx

@@ -34,6 +34,7 @@ struct Schedule {
     bool binned = false;
     bool overlap = false;  // double-buffered photon rounds (bin passes on a second stream)
     uint32_t cam_blocks = 0;  // > 0: the camera round runs concurrently, on its own stream
+    uint32_t ph_threads = 256;  // photon-kernel block size (1024: one block per CU)
     std::vector<BatchJob> table;
     std::vector<BatchItem> ordered;
     std::vector<BatchQueues> qh;   // per round; round 0 = camera items
@@ -273,6 +274,7 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     for (uint32_t q = 0; q < kXcdQueues; ++q) cam_items += S.qh[0].size[q];
     ph_items = (uint32_t)S.ordered.size() - cam_items;
     S.cam_blocks = T.camera_blocks && cam_items && ph_items ? T.camera_blocks : 0;
+    S.ph_threads = (S.splat_mode == 1 || S.splat_mode == 3) && T.photon_threads == 1024 && S.blocks >= 4 ? 1024 : 256;
     S.qstride = a256(sizeof(BatchQueues));
     S.off_items = S.qstride * S.n_rounds;
     S.off_jobs = a256(S.off_items + S.ordered.size() * sizeof(BatchItem));
@@ -366,7 +368,7 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
         const bool camera = r == 0;
         return timed(ctx, timing, camera ? SPX_KT_CAMERA : SPX_KT_PHOTON, st, [&] {
             return launch_batch(camera, (const BatchJob *)(d + S.off_jobs), (const BatchItem *)(d + S.off_items),
-                                (BatchQueues *)(d + r * S.qstride), blocks, S.splat_mode, bst, st);
+                                (BatchQueues *)(d + r * S.qstride), blocks, S.splat_mode, S.ph_threads, bst, st);
         });
     };
     auto trace = [&](uint32_t r, const BinState &bst) { return trace_on(r, bst, s, S.blocks); };
@@ -459,7 +461,7 @@ int batch_stage(spx_ctx *ctx, size_t bytes, hipStream_t s) {
 // buffer sizes depend on, apart from the uploaded per-job values.
 bool same_layout(const Schedule &a, const Schedule &b) {
     if (a.n_jobs != b.n_jobs || a.n_rounds != b.n_rounds || a.binned != b.binned || a.overlap != b.overlap ||
-        a.splat_mode != b.splat_mode || a.cam_blocks != b.cam_blocks ||
+        a.splat_mode != b.splat_mode || a.cam_blocks != b.cam_blocks || a.ph_threads != b.ph_threads ||
         a.blocks != b.blocks || a.bytes != b.bytes || a.rep_total != b.rep_total || a.bin_want != b.bin_want ||
         a.ordered.size() != b.ordered.size() || a.reps.size() != b.reps.size())
         return false;
@@ -890,15 +892,19 @@ int spx_autotune(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, uint32_t sw
         std::vector<uint32_t> values;
         bool binned_only;
     };
+    // Launch-structure axes first (deposit mode, overlapped rounds, the
+    // concurrent camera grid, then the persistent grid it shares the GPU
+    // with), then the work-item and tile sizes.
     const std::vector<Axis> axes = {
         {&spx_tuning::splat_mode, {2, 3, 1}, false},
+        {&spx_tuning::overlap_bins, {0, 1}, true},
+        {&spx_tuning::camera_blocks, {0, cus / 2, cus, cus * 2}, false},
+        {&spx_tuning::batch_blocks, {cus * 2, cus * 3, cus * 4, cus * 8}, false},
         {&spx_tuning::chunk_photons, {2048, 4096, 8192, 16384, 32768}, false},
         {&spx_tuning::chunk_paths, {4096, 8192, 16384, 32768}, false},
-        {&spx_tuning::batch_blocks, {cus * 2, cus * 4, cus * 8}, false},
         {&spx_tuning::tile_wlog2, {3, 4, 5, 6, 7}, true},
         {&spx_tuning::bin_log2, {27, 28, 29, 30}, true},
-        {&spx_tuning::overlap_bins, {0, 1}, true},
-        {&spx_tuning::camera_blocks, {0, cus, cus * 2}, false},
+        {&spx_tuning::photon_threads, {256, 1024}, false},
     };
     spx_tuning best_t = start;
     if (best_t.batch_blocks == 0) best_t.batch_blocks = cus * 4;

@@ -19,14 +19,17 @@ def _small_mix(n=16):
     return jobs.mixed_jobs(n=n, seed=3, max_side=160, max_photons_log2=16, max_spp=12)
 
 
-@pytest.mark.parametrize("splat_mode,bin_log2", [(0, 27), (1, 27), (2, 27), (2, 15), (3, 27), (3, 15)])
-def test_batch_equals_alone(ctx, splat_mode, bin_log2):
+@pytest.mark.parametrize("splat_mode,bin_log2,threads", [(0, 27, 256), (1, 27, 256), (2, 27, 256), (2, 15, 256),
+                                                          (3, 27, 256), (3, 15, 256), (1, 27, 1024), (3, 27, 1024),
+                                                          (3, 15, 1024)])
+def test_batch_equals_alone(ctx, splat_mode, bin_log2, threads):
     # bin_log2 = 15 (32768 records per round) forces many rounds and pool
-    # overflow into the atomic fallback.
+    # overflow into the atomic fallback; threads = 1024 runs the hash modes
+    # with one 8192-slot table per compute unit.
     specs = _small_mix()
     br = jobs.BatchRunner(ctx, specs)
     old = ctx.tuning()
-    ctx.set_tuning(bin_log2=bin_log2)
+    ctx.set_tuning(bin_log2=bin_log2, photon_threads=threads)
     try:
         br.clear()
         br.run_batched(cfg=A.BatchCfg(splat_mode, 0, 4096, 2048))
@@ -39,7 +42,7 @@ def test_batch_equals_alone(ctx, splat_mode, bin_log2):
             err = rel_l1_per_channel(a, b)
             assert (err < ALONE_TOL).all(), f"job {j} ({specs[j]}): {err}"
     finally:
-        ctx.set_tuning(bin_log2=old.bin_log2)
+        ctx.set_tuning(bin_log2=old.bin_log2, photon_threads=old.photon_threads)
         br.close()
 
 

@@ -23,16 +23,19 @@ def _images(br):
     return [f.image().copy() for f in br.films]
 
 
-@pytest.mark.parametrize("splat_mode,overlap", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
-def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap):
+@pytest.mark.parametrize("splat_mode,overlap,threads", [(1, 0, 256), (2, 0, 256), (3, 0, 256), (2, 1, 256),
+                                                         (3, 1, 256), (3, 1, 1024)])
+def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap, threads):
     # overlap = 1: the graph forks the bin passes onto a second branch
     # (double-buffered rounds; bin_log2 = 21 makes several rounds without
-    # overflowing the record pools, whose fallback atomics would reorder sums)
+    # overflowing the record pools, whose fallback atomics would reorder sums);
+    # threads = 1024: the photon kernel in compute-unit-wide blocks
     specs = _small_mix()
     br = jobs.BatchRunner(ctx, specs)
     cfg = A.BatchCfg(splat_mode, 0, 4096, 2048)
     plan = None
     old = ctx.tuning()
+    ctx.set_tuning(photon_threads=threads)
     if overlap:  # and the camera round on a third branch
         ctx.set_tuning(overlap_bins=1, bin_log2=21, camera_blocks=256)
     try:
