@@ -4,8 +4,9 @@ One step = the mixed stream of 512 uneven jobs (half rainbow camera renders,
 half prism-on-floor volume renders; spectral_amd.jobs.mixed_jobs, seed 0)
 traced by the batched engine, inputs resident in HBM: the table's schedule is
 captured once as a hipGraph (spx_batch_plan_create) and each step replays it.
-The launch shape is picked first by the autotuner (spx_autotune) on the same
-table.  value = (paths + photons) per second / 1e6 over exactly K timed steps.
+The launch shape is the autotuner's pick on this table (spx_autotune,
+recorded in TUNED_MIXED512; --autotune runs the search again first).
+value = (paths + photons) per second / 1e6 over exactly K timed steps.
 
 Also reported on the same line: the plain one-job-per-launch form on the same
 stream, the eager (ungraphed) step, per-kernel times of the batched step (HIP
@@ -17,7 +18,8 @@ dominant kernel, the per-configuration throughputs (DSOTM prism 2^20 photons at
 served stream's p50/p99 job latency and the CPU oracle timed on this host.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
-                       [--no-tune | --tuning JSON] [--no-cpu] [--no-configs] [--no-serve] [--no-plain]
+                       [--autotune | --no-tune | --tuning JSON] [--sequential]
+                       [--no-cpu] [--no-configs] [--no-serve] [--no-plain]
 """
 from __future__ import annotations
 
@@ -34,6 +36,13 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "Mphotons/s + Mpaths/s on 1 MI355X, batched vs plain; p50/p99 job latency"
+# spx_autotune's pick on the 512-job mixed stream (bench.py --autotune, one
+# coordinate sweep; repeated runs pick 768 or 1024 persistent blocks and
+# 4096- or 8192-photon items within 2 % of each other): deposit mode 3,
+# double-buffered rounds, a 128-block concurrent camera round.
+TUNED_MIXED512 = {"max_replicas": 16, "replica_mib": 64, "splat_mode": 3, "chunk_log2": 22, "bin_log2": 30,
+                  "tile_wlog2": 5, "batch_blocks": 768, "chunk_paths": 16384, "chunk_photons": 4096,
+                  "overlap_bins": 1, "camera_blocks": 128, "photon_threads": 256}
 UNIT = "Mphotons/s + Mpaths/s"
 HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes (per unit of work): a camera path segment moves a 44 B
@@ -229,6 +238,13 @@ def per_config(ctx, timer, quick):
     return out
 
 
+def photon_kernel_name(t) -> str:
+    """The photon trace kernel a tuning launches, as rocprofv3 names it."""
+    mode = t["splat_mode"]
+    wide = t.get("photon_threads", 256) == 1024 and mode in (1, 3)
+    return f"k_batch_photons<{mode}, {1024 if wide else 256}>"
+
+
 def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
     """Algorithmic HBM bytes of a kernel group over its measured time."""
     per_launch = alg_bytes_step / max(launches_step, 1)
@@ -243,6 +259,10 @@ def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
     busy = profile_figure("batched_step_mixed512_sq_counters.md", f"valu_busy[{name}]")
     if busy is not None:
         out["valu_busy"] = busy
+    # and how many of a wave's 64 lanes take part in those instructions
+    lanes = profile_figure("batched_step_mixed512_sq_counters.md", f"lanes_active[{name}]")
+    if lanes is not None:
+        out["lanes_active"] = lanes
     return out
 
 
@@ -266,7 +286,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--quick", action="store_true", help="smaller per-config sizes")
-    ap.add_argument("--no-tune", action="store_true", help="keep the default tuning (no autotuner)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run spx_autotune on the table first instead of installing TUNED_MIXED512")
+    ap.add_argument("--no-tune", action="store_true", help="keep the context's default tuning")
+    ap.add_argument("--sequential", action="store_true",
+                    help="rounds in sequence throughout (no overlapped bin passes, no concurrent camera "
+                         "round): every launch runs alone, as profilers should see it")
     ap.add_argument("--tuning", type=str, default=None,
                     help="JSON object of spx_tuning fields to install instead of autotuning")
     ap.add_argument("--no-cpu", action="store_true")
@@ -303,16 +328,24 @@ def main() -> int:
     log(f"mixed stream: {len(specs)} jobs, {paths / 1e9:.3f} G paths, {photons / 1e9:.3f} G photons")
     br = jobs.BatchRunner(ctx, specs)
 
-    # launch shape: explicit, default, or autotuned on the step's own table
-    tune_info = {"source": "default"}
+    # launch shape: explicit, the context default, autotuned now, or the
+    # autotuner's recorded pick
     if args.tuning:
         ctx.set_tuning(**json.loads(args.tuning))
         tune_info = {"source": "given"}
-    elif not args.no_tune:
+    elif args.no_tune:
+        tune_info = {"source": "context default"}
+    elif args.autotune:
         t0 = time.perf_counter()
         _, tune_ms = br.autotune(stream, sweeps=1)
         tune_info = {"source": "spx_autotune on the 512-job table", "seconds": round(time.perf_counter() - t0, 1),
                      "tuned_ms": round(tune_ms, 3)}
+    else:
+        ctx.set_tuning(**TUNED_MIXED512)
+        tune_info = {"source": "spx_autotune's pick on the 512-job table (bench.py --autotune)"}
+    if args.sequential:
+        ctx.set_tuning(overlap_bins=0, camera_blocks=0)
+        tune_info["sequential"] = True
     tuning = ctx.tuning().as_dict()
     log(f"tuning ({tune_info['source']}): {tuning}")
 
@@ -359,7 +392,7 @@ def main() -> int:
     steps = args.steps
     groups = {
         "camera": kernel_roofline("k_batch_camera", cam_bytes, kt["camera"][0] / steps, kt["camera"][1] // steps),
-        "photon": kernel_roofline(f"k_batch_photons<{tuning['splat_mode']}>", ph_bytes, kt["photon"][0] / steps,
+        "photon": kernel_roofline(photon_kernel_name(tuning), ph_bytes, kt["photon"][0] / steps,
                                   kt["photon"][1] // steps),
     }
     if kt["bins"][1]:
@@ -376,13 +409,15 @@ def main() -> int:
     dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
     roofline = dict(groups[dominant])
     roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
-                                                 "frac", "traffic", "valu_busy", "atomics_gops", "atomic_frac")
+                                                 "frac", "traffic", "valu_busy", "lanes_active", "atomics_gops",
+                                                 "atomic_frac")
                                if k in v}
                            for g, v in groups.items()}
     roofline["atomic_peak_gops"] = round(a_peak, 2)
     roofline["note"] = ("achieved = algorithmic bytes (wavefront path state + deposits) / kernel time; the "
                         "persistent kernels keep path state in registers, so measured HBM traffic is the "
-                        "smaller `traffic` and the bound is VALU issue (`valu_busy`)")
+                        "smaller `traffic` and the bound is VALU issue (`valu_busy`), with `lanes_active` of a "
+                        "wave's 64 lanes working on average")
     roofline["work_model"] = {k: round(v, 4) for k, v in wm.items()}
     roofline["work_model_source"] = "oracle counts on the cpu_baseline samples" if cpu else "pinned constants"
     result = {

@@ -186,7 +186,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__res
         const BatchItem item = items[it];
         const uint32_t j = uniform_u32(item.job);
         const uint32_t kind = uniform_u32(jobs[j].kind);
-        const PhotonParams P = jobs[j].ph;
+        const PhotonParams &P = jobs[j].ph;
         stage_for(B, P.scene, P.tbl);
         if constexpr (SPLAT == 0) {
             GlobalSink g{deposit_film(P)};
