@@ -29,6 +29,10 @@ constexpr uint32_t kMaxCamPix = 2048;  // LDS pixel accumulator capacity per ite
 __device__ inline uint32_t uniform_u32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Camera item: pixels [start, start+count) of job J, all spp samples each.
+// Unit f is sample f / count of item pixel f % count: the lanes of a wave
+// work on neighbouring pixels, so their LDS sums never contend for one
+// address, and when count divides the block size each lane keeps one pixel
+// and sums its samples in sample order (as the reference does).
 __device__ inline void batch_camera_item(const SceneData &S, const float4 *tbl, const CamParams &P,
                                          uint32_t start, uint32_t count, float *acc) {
     for (uint32_t k = threadIdx.x; k < 3 * count; k += blockDim.x) acc[k] = 0.0f;
@@ -37,8 +41,8 @@ __device__ inline void batch_camera_item(const SceneData &S, const float4 *tbl, 
     uint32_t f = threadIdx.x;
     if (f < total) {
         CamState st;
-        uint32_t lp = f / P.spp;
-        camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f % P.spp, st);
+        uint32_t lp = f % count;
+        camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f / count, st);
         uint32_t b = 0;
         while (true) {
             bool alive = camera_bounce(S, tbl, P, b, st);
@@ -50,8 +54,8 @@ __device__ inline void batch_camera_item(const SceneData &S, const float4 *tbl, 
             __hip_atomic_fetch_add(acc + 3 * lp + 2, c.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
             f += blockDim.x;
             if (f >= total) break;
-            lp = f / P.spp;
-            camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f % P.spp, st);
+            lp = f % count;
+            camera_begin(P, (start + lp) % P.w, (start + lp) / P.w, P.sample_base + f / count, st);
             b = 0;
         }
     }
