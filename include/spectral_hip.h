@@ -265,6 +265,31 @@ int spx_surface_rim(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam,
 int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint32_t height,
                       float scale, float exposure, uint8_t *rgb, void *stream);
 
+/* The viewer's progressive display pass (spectral-viewer main.rs:131-174,
+ * the blit shader; its fixed-point accum is an FP32 film here, so its SCALE
+ * is 1).  Per pixel: the Gaussian-weighted mean of vol_sum over the
+ * (2r+1)^2 neighbourhood inside the film, r = (int)kradius, sigma =
+ * max(kradius / 2, 0.5), times target_n / max(total_photons, 1); plus the
+ * surface layer (3*w*h XYZ, NULL = none); times exposure; XYZ -> linear
+ * sRGB, Reinhard c/(1+c), clamp below at 0, gamma 1/2.2.  rgba (4*w*h
+ * bytes) receives round(255 * clamp(v, 0, 1)) per channel (the Rgba8Unorm
+ * store) and alpha 255.  All buffers are device memory; row 0 = top. */
+int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
+                        uint32_t height, uint32_t total_photons, float target_n, float exposure,
+                        float kradius, uint8_t *rgba, void *stream);
+
+/* The exposure rules' luminance quantile (viewer compute_exposure_p997,
+ * main.rs:614-644 with q = 0.997; prism_on_plane.rs:172-176 with q = 0.995
+ * and total_photons = target_n = 1): with L_i = vol_sum_Y[i] * (target_n /
+ * max(total_photons, 1)) + surface_Y[i] (surface may be NULL), *lum_q =
+ * max(L_(k), 1e-9) where L_(k) is the k-th smallest, k = min(floor(fp32(w*h)
+ * * q), w*h - 1).  The caller's exposure is c / *lum_q (1 in the viewer, 1.1
+ * in prism_on_plane).  Exact selection on the device (radix select over the
+ * float bits); device buffers in, host float out; synchronous. */
+int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
+                         uint32_t height, uint32_t total_photons, float target_n, float quantile,
+                         float *lum_q);
+
 /* ---- known-answer kernels ------------------------------------------------------ */
 /* PathRng::new(key, seed) then n next_u32 draws on the device (rng_parity.rs:4-16). */
 int spx_rng_draws(spx_ctx *ctx, uint32_t key, uint32_t seed, uint32_t n, uint32_t *out_host);

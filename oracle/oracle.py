@@ -68,6 +68,8 @@ _SIG = {
     "or_phase_hg": (_f32, [_f32, _f32]),
     "or_photon_record_states": (None, [_P(OrScene), _vp, _vp, _u32, _u32, _f32, _u32, _vp, _vp]),
     "or_surface_rim": (None, [_P(OrScene), _vp, _u32, _u32, _vp, _vp]),
+    "or_display_resolve": (None, [_vp, _vp, _u32, _u32, _u32, _f32, _f32, _f32, _vp]),
+    "or_display_quantile": (_f32, [_vp, _vp, _u32, _u32, _u32, _f32, _f32]),
     "or_struct_sizes": (_u32, [_u32]),
 }
 
@@ -147,6 +149,25 @@ def photon_record(scene, bm, idx, seed, max_dist, mode, opts, scr=None):
                                   None if scr is None else C.byref(scr), idx, seed, max_dist, mode,
                                   C.byref(opts), C.byref(rec))
     return rec
+
+
+def display_resolve(vol, surf, total_photons, target_n, exposure, kradius):
+    """The viewer's display pass on (h, w, 3) float32 films -> (h, w, 4) uint8."""
+    vol = np.ascontiguousarray(vol, np.float32)
+    h, w = vol.shape[:2]
+    sv = None if surf is None else np.ascontiguousarray(surf, np.float32)
+    out = np.zeros((h, w, 4), np.uint8)
+    lib().or_display_resolve(vol.ctypes.data, None if sv is None else sv.ctypes.data, w, h, total_photons,
+                             target_n, exposure, kradius, out.ctypes.data)
+    return out
+
+
+def display_quantile(vol, surf, total_photons, target_n, quantile):
+    vol = np.ascontiguousarray(vol, np.float32)
+    h, w = vol.shape[:2]
+    sv = None if surf is None else np.ascontiguousarray(surf, np.float32)
+    return lib().or_display_quantile(vol.ctypes.data, None if sv is None else sv.ctypes.data, w, h,
+                                     total_photons, target_n, quantile)
 
 
 def surface_rim(scene, cam, w, h):

@@ -1189,6 +1189,80 @@ void or_surface_rim(const or_scene *s, const or_camera *cam, uint32_t w, uint32_
         }
 }
 
+/* ------------------------------------------------------------------------- */
+/* Viewer display (spectral-viewer main.rs).  The viewer's accum is u32      */
+/* fixed point divided by SCALE; the films here are FP32 sums, so SCALE = 1  */
+/* and k = (1 / SCALE) * inv_n * TARGET_N = inv_n * target_n.               */
+/* ------------------------------------------------------------------------- */
+static float display_k(uint32_t total_photons, float target_n) {
+    float inv_n = 1.0f / (float)(total_photons > 1u ? total_photons : 1u); /* main.rs:144 */
+    return inv_n * target_n;                                                  /* main.rs:145 */
+}
+
+/* fs_main, main.rs:131-174: Gaussian gather, composite, exposure, XYZ ->
+ * linear sRGB (main.rs:119-124), tonemap (main.rs:126-130: Reinhard then
+ * max(.,0) then pow 1/2.2); the Rgba8Unorm target stores
+ * round(255 * clamp(v, 0, 1)). */
+void or_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, uint32_t total_photons,
+                        float target_n, float exposure, float kradius, uint8_t *rgba) {
+    const float k = display_k(total_photons, target_n);
+    const int radius = (int)kradius;                       /* main.rs:146 */
+    const float sigma = fmaxf(kradius * 0.5f, 0.5f);       /* main.rs:147 */
+    const float inv2s2 = 1.0f / (2.0f * sigma * sigma);    /* main.rs:148 */
+    for (uint32_t py = 0; py < h; ++py)
+        for (uint32_t px = 0; px < w; ++px) {
+            float acc[3] = {0.0f, 0.0f, 0.0f}, wsum = 0.0f;
+            for (int dy = -radius; dy <= radius; ++dy)     /* main.rs:151-165 */
+                for (int dx = -radius; dx <= radius; ++dx) {
+                    int sx = (int)px + dx, sy = (int)py + dy;
+                    if (sx < 0 || sx >= (int)w || sy < 0 || sy >= (int)h) continue;
+                    size_t s = (size_t)sy * w + (size_t)sx;
+                    float g = expf(-(float)(dx * dx + dy * dy) * inv2s2);
+                    for (int c = 0; c < 3; ++c) acc[c] = acc[c] + vol[3 * s + c] * g;
+                    wsum = wsum + g;
+                }
+            size_t i = (size_t)py * w + px;
+            float f = k / fmaxf(wsum, 1e-6f);              /* main.rs:168 */
+            float xyz[3], lin[3];
+            for (int c = 0; c < 3; ++c) {
+                xyz[c] = acc[c] * f;
+                if (surf) xyz[c] = xyz[c] + surf[3 * i + c];
+                xyz[c] = xyz[c] * exposure;                /* main.rs:171 */
+            }
+            or_xyz_to_linear_srgb(xyz, lin);
+            for (int c = 0; c < 3; ++c) {
+                float t = fmaxf(lin[c] / (1.0f + lin[c]), 0.0f);
+                float v = powf(t, 1.0f / 2.2f);
+                rgba[4 * i + c] = (uint8_t)(clampf(v, 0.0f, 1.0f) * 255.0f + 0.5f);
+            }
+            rgba[4 * i + 3] = 255;
+        }
+}
+
+static int cmp_float(const void *a, const void *b) {
+    float x = *(const float *)a, y = *(const float *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+/* compute_exposure_p997, main.rs:635-643 (and prism_on_plane.rs:172-176 with
+ * k = 1): the composited luminance's quantile, floored at 1e-9. */
+float or_display_quantile(const float *vol, const float *surf, uint32_t w, uint32_t h, uint32_t total_photons,
+                          float target_n, float quantile) {
+    const size_t n = (size_t)w * h;
+    const float k = display_k(total_photons, target_n);
+    float *lums = (float *)malloc(n * sizeof(float));
+    for (size_t i = 0; i < n; ++i) {
+        lums[i] = vol[3 * i + 1] * k;
+        if (surf) lums[i] = lums[i] + surf[3 * i + 1];
+    }
+    qsort(lums, n, sizeof(float), cmp_float);
+    size_t idx = (size_t)((float)n * quantile);
+    if (idx > n - 1) idx = n - 1;
+    float p = lums[idx];
+    free(lums);
+    return fmaxf(p, 1e-9f);
+}
+
 uint32_t or_struct_sizes(uint32_t which) {
     switch (which) {
     case 0: return sizeof(or_prim);

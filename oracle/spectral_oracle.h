@@ -203,6 +203,12 @@ void or_photon_record_states(const or_scene *s, const or_beam *beam, const or_sc
 void or_surface_rim(const or_scene *s, const or_camera *cam, uint32_t w, uint32_t h,
                     float *rim_xyz, float *depth);
 
+/* The viewer's display pass and exposure quantile (spectral-viewer
+ * main.rs:131-174, 614-644) on an FP32 film (SCALE = 1). */
+void or_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, uint32_t total_photons,
+                        float target_n, float exposure, float kradius, uint8_t *rgba);
+float or_display_quantile(const float *vol, const float *surf, uint32_t w, uint32_t h, uint32_t total_photons,
+                          float target_n, float quantile);
 uint32_t or_struct_sizes(uint32_t which);
 
 #ifdef __cplusplus

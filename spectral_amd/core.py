@@ -398,6 +398,44 @@ def surface_rim(ctx: Context, scene, cam: Camera, width: int, height: int):
     return rim, depth
 
 
+def resolve_srgb8(ctx: Context, film: "Film", scale: float, exposure: float) -> np.ndarray:
+    """xyz_to_linear_srgb + tonemap_to_u8 (cie.rs:121-144) of film * scale:
+    (h, w, 3) uint8."""
+    out = DeviceBuffer(ctx, (3 * film.width * film.height + 3) // 4)
+    try:
+        _check(ctx, A.lib().spx_resolve_srgb8(ctx.handle, film.ptr, film.width, film.height, scale, exposure,
+                                              out.ptr, None))
+        raw = out.download()
+    finally:
+        out.free()
+    return raw.view(np.uint8)[:3 * film.width * film.height].reshape(film.height, film.width, 3).copy()
+
+
+def display_resolve(ctx: Context, vol: "Film", surface: Optional["Film"], total_photons: int, target_n: float,
+                    exposure: float, kradius: float = 2.0) -> np.ndarray:
+    """The viewer's display pass (spectral-viewer main.rs:131-174) of a device
+    film: (h, w, 4) uint8 RGBA."""
+    out = DeviceBuffer(ctx, vol.width * vol.height)
+    try:
+        _check(ctx, A.lib().spx_display_resolve(ctx.handle, vol.ptr, surface.ptr if surface is not None else None,
+                                                vol.width, vol.height, total_photons, target_n, exposure,
+                                                kradius, out.ptr, None))
+        raw = out.download()
+    finally:
+        out.free()
+    return raw.view(np.uint8).reshape(vol.height, vol.width, 4).copy()
+
+
+def display_quantile(ctx: Context, vol: "Film", surface: Optional["Film"], total_photons: int, target_n: float,
+                     quantile: float = 0.997) -> float:
+    """The exposure rules' luminance quantile (main.rs:614-644): exposure = c / this."""
+    q = C.c_float(0.0)
+    _check(ctx, A.lib().spx_display_quantile(ctx.handle, vol.ptr, surface.ptr if surface is not None else None,
+                                             vol.width, vol.height, total_photons, target_n, quantile,
+                                             C.byref(q)))
+    return q.value
+
+
 def rng_draws(ctx: Context, key: int, seed: int, n: int) -> np.ndarray:
     out = (C.c_uint32 * n)()
     _check(ctx, A.lib().spx_rng_draws(ctx.handle, key, seed, n, out))

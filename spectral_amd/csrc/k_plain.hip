@@ -198,6 +198,95 @@ hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float
     return hipGetLastError();
 }
 
+// The viewer's blit pass on an FP32 film (spectral-viewer main.rs:131-174):
+// Gaussian-weighted mean over the (2r+1)^2 neighbourhood inside the film,
+// times k = target_n / max(total_photons, 1); plus the surface layer; times
+// exposure; XYZ -> linear sRGB (the shader's matrix, cie.rs:121-128);
+// Reinhard, clamp below at 0, gamma 1/2.2; the Rgba8Unorm store rounds
+// 255 * clamp(v, 0, 1) to the nearest byte.  One thread per pixel.
+__global__ __launch_bounds__(256) void k_display_resolve(const float *vol, const float *surf, uint32_t w,
+                                                         uint32_t h, float k, int radius, float inv2s2,
+                                                         float exposure, uint8_t *rgba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const int px = (int)(i % w), py = (int)(i / w);
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, wsum = 0.0f;
+    for (int dy = -radius; dy <= radius; ++dy)
+        for (int dx = -radius; dx <= radius; ++dx) {
+            const int sx = px + dx, sy = py + dy;
+            if (sx < 0 || sx >= (int)w || sy < 0 || sy >= (int)h) continue;
+            const size_t s = (size_t)sy * w + (size_t)sx;
+            const float g = expf(-(float)(dx * dx + dy * dy) * inv2s2);
+            ax += vol[3 * s + 0] * g;
+            ay += vol[3 * s + 1] * g;
+            az += vol[3 * s + 2] * g;
+            wsum += g;
+        }
+    const float f = k / fmaxf(wsum, 1e-6f);
+    float x = ax * f, y = ay * f, z = az * f;
+    if (surf) {
+        x += surf[3 * i + 0];
+        y += surf[3 * i + 1];
+        z += surf[3 * i + 2];
+    }
+    x *= exposure;
+    y *= exposure;
+    z *= exposure;
+    const float lin[3] = {3.2406f * x - 1.5372f * y - 0.4986f * z, -0.9689f * x + 1.8758f * y + 0.0415f * z,
+                          0.0557f * x - 0.2040f * y + 1.0570f * z};
+    uchar4 o;
+    uint8_t *oc = &o.x;
+    for (int c = 0; c < 3; ++c) {
+        const float t = fmaxf(lin[c] / (1.0f + lin[c]), 0.0f);
+        const float v = powf(t, 1.0f / 2.2f);
+        oc[c] = (uint8_t)((v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v)) * 255.0f + 0.5f);
+    }
+    o.w = 255;
+    reinterpret_cast<uchar4 *>(rgba)[i] = o;
+}
+
+hipError_t launch_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, float k,
+                                  int radius, float inv2s2, float exposure, uint8_t *rgba, hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_display_resolve, dim3((n + 255) / 256), dim3(256), 0, s, vol, surf, w, h, k, radius,
+                       inv2s2, exposure, rgba);
+    return hipGetLastError();
+}
+
+// Radix-select pass over the composited luminance L_i = vol_Y[i] * k
+// (+ surf_Y[i]): a 256-bin histogram of the key digit at `shift` among the
+// keys whose bits under `mask` equal `prefix`.  Keys order like the floats
+// (sign bit flipped for non-negative values, all bits for negative ones).
+__device__ inline uint32_t float_order_key(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__global__ __launch_bounds__(256) void k_lum_digit_hist(const float *vol, const float *surf, uint32_t n, float k,
+                                                        uint32_t prefix, uint32_t mask, uint32_t shift,
+                                                        uint32_t *hist) {
+    __shared__ uint32_t sh[256];
+    sh[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float L = vol[3 * (size_t)i + 1] * k;
+        if (surf) L = L + surf[3 * (size_t)i + 1];
+        const uint32_t key = float_order_key(L);
+        if ((key & mask) == prefix) atomicAdd(&sh[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], sh[threadIdx.x]);
+}
+
+hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n, float k, uint32_t prefix,
+                                 uint32_t mask, uint32_t shift, uint32_t *hist, hipStream_t s) {
+    uint32_t blocks = (n + 255) / 256;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    hipLaunchKernelGGL(k_lum_digit_hist, dim3(blocks), dim3(256), 0, s, vol, surf, n, k, prefix, mask, shift,
+                       hist);
+    return hipGetLastError();
+}
+
 // Known-answer kernels ---------------------------------------------------------
 __global__ void k_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;

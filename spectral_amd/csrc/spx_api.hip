@@ -777,6 +777,66 @@ int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint3
                      "spx_resolve_srgb8");
 }
 
+// k = (1 / SCALE) * inv_n * TARGET_N of the viewer with SCALE = 1 (FP32 film).
+static float display_k(uint32_t total_photons, float target_n) {
+    const float inv_n = 1.0f / (float)(total_photons > 1u ? total_photons : 1u);
+    return inv_n * target_n;
+}
+
+int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width, uint32_t height,
+                        uint32_t total_photons, float target_n, float exposure, float kradius, uint8_t *rgba,
+                        void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    int rc;
+    if ((rc = check_film(ctx, vol_sum, width, height, "spx_display_resolve"))) return rc;
+    if (!rgba) return set_error(ctx, SPX_EINVAL, "spx_display_resolve: null output");
+    if (!std::isfinite(target_n) || !std::isfinite(exposure) || !(kradius >= 0.0f) || !(kradius <= 32.0f))
+        return set_error(ctx, SPX_EINVAL, "spx_display_resolve: bad target_n, exposure or kradius (0..32)");
+    const float sigma = fmaxf(kradius * 0.5f, 0.5f);
+    const float inv2s2 = 1.0f / (2.0f * sigma * sigma);
+    return check_hip(ctx,
+                     launch_display_resolve(vol_sum, surface, width, height, display_k(total_photons, target_n),
+                                            (int)kradius, inv2s2, exposure, rgba, (hipStream_t)stream),
+                     "spx_display_resolve");
+}
+
+int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width, uint32_t height,
+                         uint32_t total_photons, float target_n, float quantile, float *lum_q) {
+    if (!ctx) return SPX_EINVAL;
+    int rc;
+    if ((rc = check_film(ctx, vol_sum, width, height, "spx_display_quantile"))) return rc;
+    if (!lum_q || !std::isfinite(target_n) || !(quantile >= 0.0f) || !(quantile <= 1.0f))
+        return set_error(ctx, SPX_EINVAL, "spx_display_quantile: bad output, target_n or quantile (0..1)");
+    const uint32_t n = width * height;
+    uint64_t rank = (uint64_t)((float)n * quantile);
+    if (rank > n - 1u) rank = n - 1u;
+    const float k = display_k(total_photons, target_n);
+    uint32_t *hist = nullptr;
+    hipError_t e = hipMalloc(&hist, 256 * sizeof(uint32_t));
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile alloc");
+    uint32_t prefix = 0, mask = 0, h_hist[256];
+    for (int pass = 0; pass < 4 && e == hipSuccess; ++pass) {
+        const uint32_t shift = 24u - 8u * (uint32_t)pass;
+        e = hipMemset(hist, 0, 256 * sizeof(uint32_t));
+        if (e == hipSuccess) e = launch_lum_digit_hist(vol_sum, surface, n, k, prefix, mask, shift, hist, nullptr);
+        if (e == hipSuccess) e = hipMemcpy(h_hist, hist, sizeof(h_hist), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
+        uint32_t b = 0;
+        uint64_t below = 0;
+        while (b < 255u && rank >= below + h_hist[b]) below += h_hist[b++];
+        rank -= below;
+        prefix |= b << shift;
+        mask |= 0xFFu << shift;
+    }
+    hipFree(hist);
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile");
+    const uint32_t u = (prefix & 0x80000000u) ? (prefix & 0x7FFFFFFFu) : ~prefix;
+    float L;
+    memcpy(&L, &u, sizeof(L));
+    *lum_q = fmaxf(L, 1e-9f);
+    return SPX_OK;
+}
+
 int spx_rng_draws(spx_ctx *ctx, uint32_t key, uint32_t seed, uint32_t n, uint32_t *out_host) {
     if (!ctx) return SPX_EINVAL;
     if (!out_host || n == 0 || n > 1u << 20) return set_error(ctx, SPX_EINVAL, "spx_rng_draws: bad arguments");

@@ -110,6 +110,10 @@ hipError_t launch_surface_rim(const SceneData *scene, const spx_camera &cam, uin
                               float *rim, float *depth, hipStream_t s);
 hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float scale,
                                 float exposure, uint8_t *rgb, hipStream_t s);
+hipError_t launch_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, float k,
+                                  int radius, float inv2s2, float exposure, uint8_t *rgba, hipStream_t s);
+hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n, float k, uint32_t prefix,
+                                 uint32_t mask, uint32_t shift, uint32_t *hist, hipStream_t s);
 hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out, hipStream_t s);
 hipError_t launch_glass_n(uint32_t glass, const float *lam, uint32_t n, float *out, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, size_t n, hipStream_t s);
