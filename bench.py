@@ -235,6 +235,18 @@ def per_config(ctx, timer, quick):
     film.free()
     sfilm.free()
     zbuf.free()
+    # the floor-caustic product image of prism_on_plane.rs at its own sizes:
+    # 4e7 caustic photons on a 512^2 floor, 3.5e7 beam-estimator photons at
+    # 800x600, every pass on the device (wall time of the whole pipeline)
+    from spectral_amd import product
+    n_c, n_v = (1 << 22, 1 << 22) if quick else (40_000_000, 35_000_000)
+    ctx.sync(st)
+    t0 = time.perf_counter()
+    img = product.prism_on_plane(ctx, 800, 600, n_c, n_v, stream=st)
+    wall = time.perf_counter() - t0
+    out["prism_on_plane_800x600_ms"] = round(wall * 1e3, 1)
+    out["prism_on_plane_800x600_mphotons_s"] = round((n_c + n_v) / wall / 1e6, 1)
+    out["prism_on_plane_800x600_exposure"] = round(img.exposure, 4)
     return out
 
 

@@ -290,6 +290,32 @@ int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surfac
                          uint32_t height, uint32_t total_photons, float target_n, float quantile,
                          float *lum_q);
 
+/* ---- the floor-caustic product image (prism_on_plane.rs:16-185) ---------------- */
+/* box_blur (prism_on_plane.rs:40-74): separable box filter of radius r over
+ * a 3*w*h XYZ buffer, clamp-to-edge; the horizontal pass into tmp, the
+ * vertical pass into out, each tap sum in tap order then times 1/(2r+1).
+ * Device buffers of 3*w*h floats; out may be in, tmp must differ from both. */
+int spx_box_blur(spx_ctx *ctx, const float *in, float *tmp, float *out, uint32_t width, uint32_t height,
+                 uint32_t radius, void *stream);
+
+/* prism_on_plane's surface pass (prism_on_plane.rs:16-38 shade_floor,
+ * 121-154): the camera ray at each pixel centre against the scene's nearest
+ * solid and the floor screen's rectangle.  The floor shades as ambient +
+ * gain * Screen::sample(floor_sum) (bilinear, lighttrace.rs:99-113), a miss
+ * as background; a solid nearer than the floor as 0.7 * the floor behind +
+ * glass_tint + (1 - |n.d|)^3 * (0.6, 0.6, 0.7).  surface (3*w*h) and depth
+ * (w*h, distance to the shaded surface, +INF on a miss: the volume pass's
+ * z-buffer) are device buffers; floor_sum is the screen's device film. */
+int spx_floor_surface(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, const spx_screen *floor,
+                      const float *floor_sum, float floor_gain, const float ambient[3], const float background[3],
+                      const float glass_tint[3], uint32_t width, uint32_t height, float *surface, float *depth,
+                      void *stream);
+
+/* tonemap_to_u8(xyz_to_linear_srgb(surface + vol), exposure) per pixel
+ * (prism_on_plane.rs:166-182, cie.rs:121-144); rgb: 3*w*h bytes, device. */
+int spx_composite_srgb8(spx_ctx *ctx, const float *surface, const float *vol, uint32_t width, uint32_t height,
+                        float exposure, uint8_t *rgb, void *stream);
+
 /* ---- known-answer kernels ------------------------------------------------------ */
 /* PathRng::new(key, seed) then n next_u32 draws on the device (rng_parity.rs:4-16). */
 int spx_rng_draws(spx_ctx *ctx, uint32_t key, uint32_t seed, uint32_t n, uint32_t *out_host);

@@ -70,6 +70,10 @@ _SIG = {
     "or_surface_rim": (None, [_P(OrScene), _vp, _u32, _u32, _vp, _vp]),
     "or_display_resolve": (None, [_vp, _vp, _u32, _u32, _u32, _f32, _f32, _f32, _vp]),
     "or_display_quantile": (_f32, [_vp, _vp, _u32, _u32, _u32, _f32, _f32]),
+    "or_box_blur": (None, [_vp, _vp, _u32, _u32, C.c_int]),
+    "or_floor_surface": (None, [_P(OrScene), _vp, _vp, _vp, _f32, _P(_f32), _P(_f32), _P(_f32), _u32, _u32, _vp,
+                                _vp]),
+    "or_composite_srgb8": (None, [_vp, _vp, _u32, _u32, _f32, _vp]),
     "or_struct_sizes": (_u32, [_u32]),
 }
 
@@ -168,6 +172,33 @@ def display_quantile(vol, surf, total_photons, target_n, quantile):
     sv = None if surf is None else np.ascontiguousarray(surf, np.float32)
     return lib().or_display_quantile(vol.ctypes.data, None if sv is None else sv.ctypes.data, w, h,
                                      total_photons, target_n, quantile)
+
+
+def box_blur(img, radius):
+    img = np.ascontiguousarray(img, np.float32)
+    out = np.empty_like(img)
+    lib().or_box_blur(img.ctypes.data, out.ctypes.data, img.shape[1], img.shape[0], radius)
+    return out
+
+
+def floor_surface(scene, cam, scr, floor_sum, gain, ambient, background, tint, w, h):
+    sv = scene_view(scene)
+    fs = np.ascontiguousarray(floor_sum, np.float32)
+    surface = np.zeros((h, w, 3), np.float32)
+    depth = np.zeros((h, w), np.float32)
+    f3 = lambda v: (C.c_float * 3)(*v)
+    lib().or_floor_surface(C.byref(sv), C.byref(cam.basis), C.byref(scr), fs.ctypes.data, gain, f3(ambient),
+                           f3(background), f3(tint), w, h, surface.ctypes.data, depth.ctypes.data)
+    return surface, depth
+
+
+def composite_srgb8(surface, vol, exposure):
+    surface = np.ascontiguousarray(surface, np.float32)
+    vol = np.ascontiguousarray(vol, np.float32)
+    out = np.zeros(vol.shape, np.uint8)
+    lib().or_composite_srgb8(surface.ctypes.data, vol.ctypes.data, vol.shape[1], vol.shape[0], exposure,
+                             out.ctypes.data)
+    return out
 
 
 def surface_rim(scene, cam, w, h):

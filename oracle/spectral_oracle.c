@@ -1263,6 +1263,107 @@ float or_display_quantile(const float *vol, const float *surf, uint32_t w, uint3
     return fmaxf(p, 1e-9f);
 }
 
+/* ------------------------------------------------------------------------- */
+/* The floor-caustic product image, prism_on_plane.rs:16-185                 */
+/* ------------------------------------------------------------------------- */
+/* box_blur, prism_on_plane.rs:40-74: horizontal then vertical, clamped. */
+void or_box_blur(const float *in, float *out, uint32_t w, uint32_t h, int radius) {
+    const float norm = 1.0f / (float)(2 * radius + 1);
+    float *tmp = (float *)malloc((size_t)3 * w * h * sizeof(float));
+    for (int pass = 0; pass < 2; ++pass) {
+        const float *src = pass == 0 ? in : tmp;
+        float *dst = pass == 0 ? tmp : out;
+        for (int y = 0; y < (int)h; ++y)
+            for (int x = 0; x < (int)w; ++x) {
+                float acc[3] = {0.0f, 0.0f, 0.0f};
+                for (int d = -radius; d <= radius; ++d) {
+                    int xx = pass == 0 ? x + d : x, yy = pass == 0 ? y : y + d;
+                    xx = xx < 0 ? 0 : (xx > (int)w - 1 ? (int)w - 1 : xx);
+                    yy = yy < 0 ? 0 : (yy > (int)h - 1 ? (int)h - 1 : yy);
+                    const float *p = src + 3 * ((size_t)yy * w + (size_t)xx);
+                    for (int c = 0; c < 3; ++c) acc[c] += p[c];
+                }
+                for (int c = 0; c < 3; ++c) dst[3 * ((size_t)y * w + (size_t)x) + c] = acc[c] * norm;
+            }
+    }
+    free(tmp);
+}
+
+/* Screen::sample, lighttrace.rs:98-113 (texel_bilinear as in screen_splat). */
+static v3 screen_sample(const screen_t *S, const float *sum, float s, float r) {
+    float fx = clampf(s * (float)S->w - 0.5f, 0.0f, (float)S->w - 1.0f);
+    float fy = clampf((1.0f - r) * (float)S->h - 0.5f, 0.0f, (float)S->h - 1.0f);
+    uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
+    uint32_t x1 = x0 + 1 < S->w - 1 ? x0 + 1 : S->w - 1;
+    uint32_t y1 = y0 + 1 < S->h - 1 ? y0 + 1 : S->h - 1;
+    float dx = fx - (float)x0, dy = fy - (float)y0;
+    const float *a = sum + 3 * ((size_t)y0 * S->w + x0), *b = sum + 3 * ((size_t)y0 * S->w + x1);
+    const float *c = sum + 3 * ((size_t)y1 * S->w + x0), *d = sum + 3 * ((size_t)y1 * S->w + x1);
+    float o[3];
+    for (int k = 0; k < 3; ++k)
+        o[k] = a[k] * (1.0f - dx) * (1.0f - dy) + b[k] * dx * (1.0f - dy) + c[k] * (1.0f - dx) * dy +
+               d[k] * dx * dy;
+    return V(o[0], o[1], o[2]);
+}
+
+/* shade_floor, prism_on_plane.rs:16-38 */
+static v3 shade_floor(const screen_t *S, const float *sum, float gain, const float *ambient,
+                      const float *background, v3 o, v3 d, float *t) {
+    float s, r;
+    if (screen_coords(S, o, d, &s, &r, t)) {
+        v3 c = screen_sample(S, sum, s, r);
+        return V(ambient[0] + c.x * gain, ambient[1] + c.y * gain, ambient[2] + c.z * gain);
+    }
+    *t = INFINITY;
+    return vl(background);
+}
+
+/* The surface pass + z-buffer, prism_on_plane.rs:125-154. */
+void or_floor_surface(const or_scene *s, const or_camera *cam, const or_screen *floor, const float *floor_sum,
+                      float gain, const float ambient[3], const float background[3], const float tint[3],
+                      uint32_t w, uint32_t h, float *surface, float *depth) {
+    screen_t S = screen_make(floor);
+    for (uint32_t py = 0; py < h; ++py)
+        for (uint32_t px = 0; px < w; ++px) {
+            float ss = ((float)px + 0.5f) / (float)w;
+            float tt = 1.0f - ((float)py + 0.5f) / (float)h;
+            v3 o = vl(cam->origin), rd = primary_ray(cam, ss, tt);
+            hit_t hit;
+            int mi = scene_intersect(s, o, rd, &hit);
+            float t_prism = mi >= 0 ? hit.t : INFINITY;
+            float floor_t;
+            v3 fc = shade_floor(&S, floor_sum, gain, ambient, background, o, rd, &floor_t);
+            size_t idx = (size_t)py * w + px;
+            v3 out;
+            float dep;
+            if (t_prism < floor_t) {
+                float c = clampf(fabsf(dot(hit.normal, rd)), 0.0f, 1.0f);
+                float q = 1.0f - c;
+                float rim = q * (q * q); /* powi(3) */
+                out = V(fc.x * 0.7f + tint[0] + rim * 0.6f, fc.y * 0.7f + tint[1] + rim * 0.6f,
+                        fc.z * 0.7f + tint[2] + rim * 0.7f);
+                dep = t_prism;
+            } else {
+                out = fc;
+                dep = floor_t;
+            }
+            vs(surface + 3 * idx, out);
+            if (depth) depth[idx] = dep;
+        }
+}
+
+/* img = surface + vol, then tonemap_to_u8(xyz_to_linear_srgb(img), exposure):
+ * prism_on_plane.rs:166-182. */
+void or_composite_srgb8(const float *surface, const float *vol, uint32_t w, uint32_t h, float exposure,
+                        uint8_t *rgb) {
+    for (size_t i = 0; i < (size_t)w * h; ++i) {
+        float img[3], lin[3];
+        for (int c = 0; c < 3; ++c) img[c] = surface[3 * i + c] + vol[3 * i + c];
+        or_xyz_to_linear_srgb(img, lin);
+        or_tonemap_to_u8(lin, exposure, rgb + 3 * i);
+    }
+}
+
 uint32_t or_struct_sizes(uint32_t which) {
     switch (which) {
     case 0: return sizeof(or_prim);

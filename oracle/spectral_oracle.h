@@ -209,6 +209,13 @@ void or_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_
                         float target_n, float exposure, float kradius, uint8_t *rgba);
 float or_display_quantile(const float *vol, const float *surf, uint32_t w, uint32_t h, uint32_t total_photons,
                           float target_n, float quantile);
+/* The floor-caustic product image (prism_on_plane.rs:16-185). */
+void or_box_blur(const float *in, float *out, uint32_t w, uint32_t h, int radius);
+void or_floor_surface(const or_scene *s, const or_camera *cam, const or_screen *floor, const float *floor_sum,
+                      float gain, const float ambient[3], const float background[3], const float tint[3],
+                      uint32_t w, uint32_t h, float *surface, float *depth);
+void or_composite_srgb8(const float *surface, const float *vol, uint32_t w, uint32_t h, float exposure,
+                        uint8_t *rgb);
 uint32_t or_struct_sizes(uint32_t which);
 
 #ifdef __cplusplus

@@ -148,6 +148,10 @@ SIGNATURES = {
                                   _u32, _P(Opts), _P(PhotonRecord)]),
     "spx_surface_rim": (_i32, [_vp, _vp, _P(CameraBasis), _u32, _u32, _vp, _vp, _vp]),
     "spx_resolve_srgb8": (_i32, [_vp, _vp, _u32, _u32, _f32, _f32, _vp, _vp]),
+    "spx_box_blur": (_i32, [_vp, _vp, _vp, _vp, _u32, _u32, _u32, _vp]),
+    "spx_floor_surface": (_i32, [_vp, _vp, _P(CameraBasis), _P(ScreenDesc), _vp, _f32, _P(_f32), _P(_f32), _P(_f32),
+                                 _u32, _u32, _vp, _vp, _vp]),
+    "spx_composite_srgb8": (_i32, [_vp, _vp, _vp, _u32, _u32, _f32, _vp, _vp]),
     "spx_display_resolve": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _f32, _f32, _f32, _vp, _vp]),
     "spx_display_quantile": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _f32, _f32, _P(_f32)]),
     "spx_rng_draws": (_i32, [_vp, _u32, _u32, _u32, _P(_u32)]),

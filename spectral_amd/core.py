@@ -411,6 +411,44 @@ def resolve_srgb8(ctx: Context, film: "Film", scale: float, exposure: float) -> 
     return raw.view(np.uint8)[:3 * film.width * film.height].reshape(film.height, film.width, 3).copy()
 
 
+def box_blur(ctx: Context, film: "Film", radius: int, out: Optional["Film"] = None, stream=None) -> "Film":
+    """box_blur (prism_on_plane.rs:40-74) of a device film into `out` (new if None)."""
+    out = out or Film(ctx, film.width, film.height)
+    tmp = Film(ctx, film.width, film.height)
+    try:
+        _check(ctx, A.lib().spx_box_blur(ctx.handle, film.ptr, tmp.ptr, out.ptr, film.width, film.height, radius,
+                                         stream))
+        ctx.sync(stream)
+    finally:
+        tmp.free()
+    return out
+
+
+def floor_surface(ctx: Context, scene, cam: Camera, floor: A.ScreenDesc, floor_sum: "Film", gain: float,
+                  ambient, background, tint, width: int, height: int, stream=None):
+    """prism_on_plane's surface pass + z-buffer (prism_on_plane.rs:16-38, 125-154)."""
+    ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
+    surface = Film(ctx, width, height)
+    depth = DeviceBuffer(ctx, width * height)
+    f3 = lambda v: (C.c_float * 3)(*v)
+    _check(ctx, A.lib().spx_floor_surface(ctx.handle, ds.handle, C.byref(cam.basis), C.byref(floor), floor_sum.ptr,
+                                          gain, f3(ambient), f3(background), f3(tint), width, height, surface.ptr,
+                                          depth.ptr, stream))
+    return surface, depth
+
+
+def composite_srgb8(ctx: Context, surface: "Film", vol: "Film", exposure: float) -> np.ndarray:
+    """tonemap_to_u8(xyz_to_linear_srgb(surface + vol), exposure): (h, w, 3) uint8."""
+    out = DeviceBuffer(ctx, (3 * vol.width * vol.height + 3) // 4)
+    try:
+        _check(ctx, A.lib().spx_composite_srgb8(ctx.handle, surface.ptr, vol.ptr, vol.width, vol.height, exposure,
+                                                out.ptr, None))
+        raw = out.download()
+    finally:
+        out.free()
+    return raw.view(np.uint8)[:3 * vol.width * vol.height].reshape(vol.height, vol.width, 3).copy()
+
+
 def display_resolve(ctx: Context, vol: "Film", surface: Optional["Film"], total_photons: int, target_n: float,
                     exposure: float, kradius: float = 2.0) -> np.ndarray:
     """The viewer's display pass (spectral-viewer main.rs:131-174) of a device

@@ -287,6 +287,124 @@ hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n
     return hipGetLastError();
 }
 
+// box_blur, prism_on_plane.rs:40-74: one pass of the separable box filter
+// (dx = 1: along rows, else along columns), clamp-to-edge, taps summed in
+// order then scaled by 1/(2r+1).
+__global__ __launch_bounds__(256) void k_box_blur_pass(const float *in, float *out, uint32_t w, uint32_t h,
+                                                       int radius, int along_x) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const int x = (int)(i % w), y = (int)(i / w);
+    const float norm = 1.0f / (float)(2 * radius + 1);
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for (int d = -radius; d <= radius; ++d) {
+        int xx = along_x ? x + d : x, yy = along_x ? y : y + d;
+        xx = xx < 0 ? 0 : (xx > (int)w - 1 ? (int)w - 1 : xx);
+        yy = yy < 0 ? 0 : (yy > (int)h - 1 ? (int)h - 1 : yy);
+        const float *p = in + 3 * ((size_t)yy * w + (size_t)xx);
+        a0 += p[0];
+        a1 += p[1];
+        a2 += p[2];
+    }
+    out[3 * (size_t)i + 0] = a0 * norm;
+    out[3 * (size_t)i + 1] = a1 * norm;
+    out[3 * (size_t)i + 2] = a2 * norm;
+}
+
+hipError_t launch_box_blur(const float *in, float *tmp, float *out, uint32_t w, uint32_t h, uint32_t radius,
+                           hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_box_blur_pass, dim3((n + 255) / 256), dim3(256), 0, s, in, tmp, w, h, (int)radius, 1);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_box_blur_pass, dim3((n + 255) / 256), dim3(256), 0, s, tmp, out, w, h, (int)radius, 0);
+    return hipGetLastError();
+}
+
+// prism_on_plane's surface pass: shade_floor (prism_on_plane.rs:16-38) and
+// the per-pixel composite of prism and floor (prism_on_plane.rs:125-154).
+__device__ inline f3 shade_floor(const PhotonParams &F, const float *floor_sum, const FloorShade &sh, f3 o, f3 d,
+                                 float &t) {
+    float s, r;
+    if (screen_coords(F, o, d, s, r, t)) {
+        const f3 c = screen_sample(F, floor_sum, s, r);
+        return mk(sh.ambient[0] + c.x * sh.gain, sh.ambient[1] + c.y * sh.gain, sh.ambient[2] + c.z * sh.gain);
+    }
+    t = INFINITY;
+    return mk(sh.background[0], sh.background[1], sh.background[2]);
+}
+
+__global__ __launch_bounds__(256) void k_floor_surface(const SceneData *scene, spx_camera cam, PhotonParams F,
+                                                       const float *floor_sum, FloorShade sh, uint32_t w,
+                                                       uint32_t h, float *surface, float *depth) {
+    __shared__ SceneData S;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(scene);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(&S);
+    for (uint32_t k = threadIdx.x; k < sizeof(SceneData) / 4; k += blockDim.x) dst[k] = src[k];
+    __syncthreads();
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t px = i % w, py = i / w;
+    const float s = ((float)px + 0.5f) / (float)w;
+    const float t = 1.0f - ((float)py + 0.5f) / (float)h;
+    const f3 o = ld3(cam.origin), rd = primary_ray(cam, s, t);
+    Hit hit;
+    const int mi = scene_hit(S, o, rd, hit);
+    const float t_prism = mi >= 0 ? hit.t : INFINITY;
+    float floor_t;
+    const f3 fc = shade_floor(F, floor_sum, sh, o, rd, floor_t);
+    f3 out;
+    float dep;
+    if (t_prism < floor_t) {
+        float c = fabsf(dot(hit.normal, rd));
+        c = c < 0.0f ? 0.0f : (c > 1.0f ? 1.0f : c);
+        const float q = 1.0f - c;
+        const float rim = q * (q * q);  // powi(3)
+        out = mk(fc.x * 0.7f + sh.tint[0] + rim * 0.6f, fc.y * 0.7f + sh.tint[1] + rim * 0.6f,
+                 fc.z * 0.7f + sh.tint[2] + rim * 0.7f);
+        dep = t_prism;
+    } else {
+        out = fc;
+        dep = floor_t;
+    }
+    surface[3 * (size_t)i + 0] = out.x;
+    surface[3 * (size_t)i + 1] = out.y;
+    surface[3 * (size_t)i + 2] = out.z;
+    if (depth) depth[i] = dep;
+}
+
+hipError_t launch_floor_surface(const SceneData *scene, const spx_camera &cam, const PhotonParams &F,
+                                const float *floor_sum, const FloorShade &sh, uint32_t w, uint32_t h,
+                                float *surface, float *depth, hipStream_t s) {
+    hipLaunchKernelGGL(k_floor_surface, dim3((w * h + 255) / 256), dim3(256), 0, s, scene, cam, F, floor_sum, sh,
+                       w, h, surface, depth);
+    return hipGetLastError();
+}
+
+// tonemap_to_u8(xyz_to_linear_srgb(surface + vol), exposure): prism_on_plane.rs:166-182.
+__global__ void k_composite_srgb8(const float *surf, const float *vol, uint32_t n, float exposure, uint8_t *rgb) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float x = surf[3 * i] + vol[3 * i], y = surf[3 * i + 1] + vol[3 * i + 1],
+                z = surf[3 * i + 2] + vol[3 * i + 2];
+    const float lin[3] = {3.2406f * x - 1.5372f * y - 0.4986f * z, -0.9689f * x + 1.8758f * y + 0.0415f * z,
+                          0.0557f * x - 0.2040f * y + 1.0570f * z};
+    for (int c = 0; c < 3; ++c) {
+        const float v = fmaxf(lin[c] * exposure, 0.0f);
+        const float mapped = v / (1.0f + v);
+        float g = mapped <= 0.0031308f ? 12.92f * mapped : 1.055f * powf(mapped, 1.0f / 2.4f) - 0.055f;
+        g = g < 0.0f ? 0.0f : (g > 1.0f ? 1.0f : g);
+        rgb[3 * i + c] = (uint8_t)(g * 255.0f + 0.5f);
+    }
+}
+
+hipError_t launch_composite_srgb8(const float *surf, const float *vol, uint32_t w, uint32_t h, float exposure,
+                                  uint8_t *rgb, hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_composite_srgb8, dim3((n + 255) / 256), dim3(256), 0, s, surf, vol, n, exposure, rgb);
+    return hipGetLastError();
+}
+
 // Known-answer kernels ---------------------------------------------------------
 __global__ void k_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;

@@ -777,6 +777,56 @@ int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint3
                      "spx_resolve_srgb8");
 }
 
+int spx_box_blur(spx_ctx *ctx, const float *in, float *tmp, float *out, uint32_t width, uint32_t height,
+                 uint32_t radius, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    int rc;
+    if ((rc = check_film(ctx, in, width, height, "spx_box_blur"))) return rc;
+    if (!tmp || !out || tmp == in || tmp == out || radius > 1024)
+        return set_error(ctx, SPX_EINVAL, "spx_box_blur: null or aliased buffers, or radius above 1024");
+    return check_hip(ctx, launch_box_blur(in, tmp, out, width, height, radius, (hipStream_t)stream), "spx_box_blur");
+}
+
+int spx_floor_surface(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, const spx_screen *floor,
+                      const float *floor_sum, float floor_gain, const float ambient[3], const float background[3],
+                      const float glass_tint[3], uint32_t width, uint32_t height, float *surface, float *depth,
+                      void *stream) {
+    int rc = check_common(ctx, scene, "spx_floor_surface");
+    if (rc) return rc;
+    if (!cam || !floor || !ambient || !background || !glass_tint || !surface)
+        return set_error(ctx, SPX_EINVAL, "spx_floor_surface: null argument");
+    if ((rc = check_film(ctx, floor_sum, floor->width, floor->height, "spx_floor_surface (floor)"))) return rc;
+    if ((rc = check_film(ctx, surface, width, height, "spx_floor_surface"))) return rc;
+    if (!finite3(cam->origin) || !finite3(cam->lower_left) || !finite3(cam->horizontal) || !finite3(cam->vertical) ||
+        !finite3(floor->corner) || !finite3(floor->u) || !finite3(floor->v) || !std::isfinite(floor_gain) ||
+        !finite3(ambient) || !finite3(background) || !finite3(glass_tint))
+        return set_error(ctx, SPX_EINVAL, "spx_floor_surface: non-finite camera, floor or shading value");
+    PhotonParams F{};
+    set_screen(F, *floor);
+    if (!(F.scr_uu > 0.0f) || !(F.scr_vv > 0.0f) || !finite3(F.scr_n))
+        return set_error(ctx, SPX_EINVAL, "spx_floor_surface: degenerate floor rectangle");
+    FloorShade sh;
+    sh.gain = floor_gain;
+    memcpy(sh.ambient, ambient, sizeof sh.ambient);
+    memcpy(sh.background, background, sizeof sh.background);
+    memcpy(sh.tint, glass_tint, sizeof sh.tint);
+    return check_hip(ctx,
+                     launch_floor_surface(scene->dev, *cam, F, floor_sum, sh, width, height, surface, depth,
+                                          (hipStream_t)stream),
+                     "spx_floor_surface");
+}
+
+int spx_composite_srgb8(spx_ctx *ctx, const float *surface, const float *vol, uint32_t width, uint32_t height,
+                        float exposure, uint8_t *rgb, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    int rc;
+    if ((rc = check_film(ctx, vol, width, height, "spx_composite_srgb8"))) return rc;
+    if (!surface || !rgb || !std::isfinite(exposure))
+        return set_error(ctx, SPX_EINVAL, "spx_composite_srgb8: null buffer or non-finite exposure");
+    return check_hip(ctx, launch_composite_srgb8(surface, vol, width, height, exposure, rgb, (hipStream_t)stream),
+                     "spx_composite_srgb8");
+}
+
 // k = (1 / SCALE) * inv_n * TARGET_N of the viewer with SCALE = 1 (FP32 film).
 static float display_k(uint32_t total_photons, float target_n) {
     const float inv_n = 1.0f / (float)(total_photons > 1u ? total_photons : 1u);

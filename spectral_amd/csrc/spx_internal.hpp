@@ -110,6 +110,20 @@ hipError_t launch_surface_rim(const SceneData *scene, const spx_camera &cam, uin
                               float *rim, float *depth, hipStream_t s);
 hipError_t launch_resolve_srgb8(const float *film, uint32_t w, uint32_t h, float scale,
                                 float exposure, uint8_t *rgb, hipStream_t s);
+// prism_on_plane's floor shading constants.
+struct FloorShade {
+    float gain;
+    float ambient[3];
+    float background[3];
+    float tint[3];
+};
+hipError_t launch_box_blur(const float *in, float *tmp, float *out, uint32_t w, uint32_t h, uint32_t radius,
+                           hipStream_t s);
+hipError_t launch_floor_surface(const SceneData *scene, const spx_camera &cam, const PhotonParams &F,
+                                const float *floor_sum, const FloorShade &sh, uint32_t w, uint32_t h,
+                                float *surface, float *depth, hipStream_t s);
+hipError_t launch_composite_srgb8(const float *surf, const float *vol, uint32_t w, uint32_t h, float exposure,
+                                  uint8_t *rgb, hipStream_t s);
 hipError_t launch_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, float k,
                                   int radius, float inv2s2, float exposure, uint8_t *rgba, hipStream_t s);
 hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n, float k, uint32_t prefix,

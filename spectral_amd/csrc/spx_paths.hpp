@@ -205,6 +205,27 @@ __device__ inline void screen_splat(const PhotonParams &P, const float4 *tbl, fl
     deposit_xy(sink, x1, y1, y1 * P.scr_w + x1, cp * w11);
 }
 
+// Screen::sample, lighttrace.rs:98-113: the bilinear read of texel_bilinear's
+// four texels, each term sum * wx * wy in the reference's order.
+__device__ inline f3 screen_sample(const PhotonParams &P, const float *sum, float s, float r) {
+    const float W = (float)P.scr_w, H = (float)P.scr_h;
+    float fx = s * W - 0.5f;
+    fx = fx < 0.0f ? 0.0f : (fx > W - 1.0f ? W - 1.0f : fx);
+    float fy = (1.0f - r) * H - 0.5f;
+    fy = fy < 0.0f ? 0.0f : (fy > H - 1.0f ? H - 1.0f : fy);
+    const uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
+    const uint32_t x1 = x0 + 1 < P.scr_w - 1 ? x0 + 1 : P.scr_w - 1;
+    const uint32_t y1 = y0 + 1 < P.scr_h - 1 ? y0 + 1 : P.scr_h - 1;
+    const float dx = fx - (float)x0, dy = fy - (float)y0;
+    const float *a = sum + 3 * ((size_t)y0 * P.scr_w + x0), *b = sum + 3 * ((size_t)y0 * P.scr_w + x1);
+    const float *c = sum + 3 * ((size_t)y1 * P.scr_w + x0), *d = sum + 3 * ((size_t)y1 * P.scr_w + x1);
+    float o[3];
+    for (int k = 0; k < 3; ++k)
+        o[k] = a[k] * (1.0f - dx) * (1.0f - dy) + b[k] * dx * (1.0f - dy) + c[k] * (1.0f - dx) * dy +
+               d[k] * dx * dy;
+    return mk(o[0], o[1], o[2]);
+}
+
 // One lighttrace bounce (trace_photon body, lighttrace.rs:133-151).
 // Returns 0 = continue, 1 = landed (splatted), 2 = lost.
 template <class Sink>

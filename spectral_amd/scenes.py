@@ -166,6 +166,28 @@ def floor_screen(w: int = 1024, h: int = 1024):
 HAZE = dict(sigma_s=0.05, sigma_t=0.05, g=0.0, max_dist=14.0)
 
 
+# prism_on_plane.rs: the floor-caustic capstone (the wedge_scene() prism on
+# a floor screen, a beam high up its face, an oblique camera).
+def plane_floor_screen(w: int = 512, h: int = 512):
+    """prism_on_plane.rs:81-87: the floor, 10 x 14 at y = -1, normal +Y."""
+    return screen((-5.0, -1.0, -5.0), (0.0, 0.0, 10.0), (14.0, 0.0, 0.0), w, h)
+
+
+def plane_beam():
+    """prism_on_plane.rs:98-103: a 1.5 x 0.12 shaft along +X entering at y ~ 0.6."""
+    return beam((-3.0, 0.55, -0.75), (0.0, 0.0, 1.5), (0.0, 0.12, 0.0), (1.0, 0.0, 0.0))
+
+
+def plane_camera(width: int = 800, height: int = 600):
+    """prism_on_plane.rs:113-119."""
+    return Camera.look_at((0.0, 4.5, 9.0), (4.0, -1.2, 0.0), (0.0, 1.0, 0.0), 50.0, width / height)
+
+
+# prism_on_plane.rs:121-123 and 157-164: shading constants and the haze.
+PLANE_SHADE = dict(background=(0.005, 0.005, 0.008), ambient=(0.03, 0.03, 0.035), tint=(0.05, 0.07, 0.09))
+PLANE_HAZE = dict(sigma_s=0.4, sigma_t=0.12, g=0.6, max_dist=22.0)
+
+
 def rainbow_scene() -> Scene:
     """Unit water sphere; D65 sun (radius 3 deg, radiance 20) 42 deg behind the
     camera at +z, elevated 42 deg; constant sky 0.05."""
