@@ -239,7 +239,9 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
 /* ---- volume (volume.rs:22-329) ---------------------------------------------- */
 /* estimator: SPX_EST_POINT (render_volumetric_scene_point), SPX_EST_BEAM
  * (render_volumetric_scene), SPX_EST_LEGACY (render_volumetric; ignores
- * sigma_t, g and zbuf).  zbuf: device, w*h floats, NULL = all +INF. */
+ * sigma_t, g and zbuf).  zbuf: device, w*h floats, NULL = all +INF.  With
+ * SPX_EST_BEAM the wavefront engine queues the photons' segments and
+ * spreads each segment's midpoint steps over a wave. */
 int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, const spx_beam *beam,
                uint32_t width, uint32_t height, uint32_t n, uint32_t photon_base,
                const spx_vol_params *vp, const float *zbuf, uint32_t seed, uint32_t estimator,

@@ -712,8 +712,18 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
     if (n == 0) return SPX_OK;
     hipStream_t s = (hipStream_t)stream;
     hipError_t e;
-    if (engine == SPX_ENGINE_PLAIN || estimator == SPX_EST_BEAM) {
+    if (engine == SPX_ENGINE_PLAIN) {
         e = launch_photon_plain(p, 0, s);
+    } else if (engine == SPX_ENGINE_WAVEFRONT && estimator == SPX_EST_BEAM) {
+        // segments queued in HBM (up to 2^chunk_log2 per chunk of photons),
+        // their steps spread over waves
+        const uint32_t depth = p.max_depth > 0 ? p.max_depth : 1;
+        const uint32_t chunk = std::min(n, std::max(1u, (1u << ctx->tuning.chunk_log2) / depth));
+        if ((rc = ensure_scratch(ctx, beam_wavefront_scratch(chunk, depth), s))) return rc;
+        e = with_replicas(ctx, p, (size_t)3 * width * height, s,
+                          [&](const PhotonParams &q) { return run_beam_wavefront(ctx, q, ctx->scratch, chunk, s); },
+                          &rc);
+        if (rc) return rc;
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
         uint32_t chunk = 1u << ctx->tuning.chunk_log2;
         if ((rc = ensure_scratch(ctx, photon_wavefront_scratch(chunk), s))) return rc;

@@ -231,6 +231,8 @@ size_t camera_wavefront_scratch(uint32_t chunk);
 hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch, uint32_t chunk,
                                 hipStream_t s);
 size_t photon_wavefront_scratch(uint32_t chunk);
+size_t beam_wavefront_scratch(uint32_t chunk, uint32_t max_depth);
+hipError_t run_beam_wavefront(spx_ctx *ctx, const PhotonParams &p, void *scratch, uint32_t chunk, hipStream_t s);
 // bins: binned deposits (hash: an LDS hash in front of the records), else
 // the LDS hash draining to film atomics.
 hipError_t run_photon_wavefront(spx_ctx *ctx, const PhotonParams &p, int lighttrace, void *scratch,

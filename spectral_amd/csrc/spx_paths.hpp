@@ -339,54 +339,64 @@ __device__ inline void splat_transverse(const PhotonParams &P, float s, float t,
         }
 }
 
-template <class Sink>
-__device__ inline void volume_beam_segment(const PhotonParams &P, const PhotonState &st, f3 cmf,
-                                           float seg_len, Sink &sink) {
-    const f3 cam_o = ld3(P.proj.o);
-    f3 p_end = st.ro + st.rd * seg_len;
+// Beam estimator, split so that a segment's steps can also be spread over
+// a wave (the wavefront beam engine): the step count of the segment from ro
+// along rd of length seg_len (volume.rs:263-276), and step k's tent splat
+// (volume.rs:277-311).
+__device__ inline uint32_t beam_step_count(const PhotonParams &P, f3 ro, f3 rd, float seg_len) {
+    f3 p_end = ro + rd * seg_len;
     float s0, t0, s1, t1;
-    uint32_t m;
-    if (project(P.proj, st.ro, s0, t0) && project(P.proj, p_end, s1, t1)) {
+    if (project(P.proj, ro, s0, t0) && project(P.proj, p_end, s1, t1)) {
         float dx = (s1 - s0) * (float)P.w;
         float dy = (t1 - t0) * (float)P.h;
         float mf = ceilf(sqrtf(dx * dx + dy * dy) * 2.0f);
         uint64_t mm = (uint64_t)mf + 1u;
-        m = (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+        return (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+    }
+    uint64_t mm = (uint64_t)ceilf(seg_len * 80.0f);
+    return (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+}
+
+template <class Sink>
+__device__ inline void beam_step(const PhotonParams &P, f3 ro, f3 rd, float power, f3 cmf, float ds, uint32_t k,
+                                 Sink &sink) {
+    const f3 cam_o = ld3(P.proj.o);
+    float dist = ((float)k + 0.5f) * ds;
+    f3 p = ro + rd * dist;
+    float s, t;
+    if (!project(P.proj, p, s, t)) return;
+    uint32_t idx = pix_index(s, t, P.w, P.h);
+    f3 to = cam_o - p;
+    float d = fmaxf(length(to), 1e-3f);
+    float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
+    if (!(d < zb)) return;
+    f3 view = to / d;
+    float cos_theta = dot(rd, view);
+    float phase = phase_hg(P.vp.g, cos_theta);
+    float trans = expf(-P.vp.sigma_t * d);
+    float weight = power * P.vp.sigma_s * phase * trans / (d * d) * ds;
+    f3 color = cmf * weight;
+    f3 perp = cross(view, mk(0.f, 1.f, 0.f));
+    if (dot(perp, perp) < 1e-8f) perp = cross(view, mk(1.f, 0.f, 0.f));
+    perp = normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
+    float s2, t2, radius_px;
+    if (project(P.proj, p + perp, s2, t2)) {
+        float dpx = (s2 - s) * (float)P.w;
+        float dpy = (t2 - t) * (float)P.h;
+        radius_px = sqrtf(dpx * dpx + dpy * dpy);
+        radius_px = radius_px < 1.5f ? 1.5f : (radius_px > 24.0f ? 24.0f : radius_px);
     } else {
-        uint64_t mm = (uint64_t)ceilf(seg_len * 80.0f);
-        m = (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
+        radius_px = 1.0f;
     }
-    float ds = seg_len / (float)m;
-    for (uint32_t k = 0; k < m; ++k) {
-        float dist = ((float)k + 0.5f) * ds;
-        f3 p = st.ro + st.rd * dist;
-        float s, t;
-        if (!project(P.proj, p, s, t)) continue;
-        uint32_t idx = pix_index(s, t, P.w, P.h);
-        f3 to = cam_o - p;
-        float d = fmaxf(length(to), 1e-3f);
-        float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
-        if (!(d < zb)) continue;
-        f3 view = to / d;
-        float cos_theta = dot(st.rd, view);
-        float phase = phase_hg(P.vp.g, cos_theta);
-        float trans = expf(-P.vp.sigma_t * d);
-        float weight = st.power * P.vp.sigma_s * phase * trans / (d * d) * ds;
-        f3 color = cmf * weight;
-        f3 perp = cross(view, mk(0.f, 1.f, 0.f));
-        if (dot(perp, perp) < 1e-8f) perp = cross(view, mk(1.f, 0.f, 0.f));
-        perp = normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
-        float s2, t2, radius_px;
-        if (project(P.proj, p + perp, s2, t2)) {
-            float dpx = (s2 - s) * (float)P.w;
-            float dpy = (t2 - t) * (float)P.h;
-            radius_px = sqrtf(dpx * dpx + dpy * dpy);
-            radius_px = radius_px < 1.5f ? 1.5f : (radius_px > 24.0f ? 24.0f : radius_px);
-        } else {
-            radius_px = 1.0f;
-        }
-        splat_transverse(P, s, t, color, radius_px, sink);
-    }
+    splat_transverse(P, s, t, color, radius_px, sink);
+}
+
+template <class Sink>
+__device__ inline void volume_beam_segment(const PhotonParams &P, const PhotonState &st, f3 cmf,
+                                           float seg_len, Sink &sink) {
+    const uint32_t m = beam_step_count(P, st.ro, st.rd, seg_len);
+    const float ds = seg_len / (float)m;
+    for (uint32_t k = 0; k < m; ++k) beam_step(P, st.ro, st.rd, st.power, cmf, ds, k, sink);
 }
 
 // One volume bounce: segment deposits, then scatter.  Returns false when

@@ -29,7 +29,7 @@ class PlaneImage:
 
 def prism_on_plane(ctx: S.Context, width: int = 800, height: int = 600, n_caustic: int = 40_000_000,
                    n_volume: int = 35_000_000, floor_side: int = 512, engine: int = A.ENGINE_PLAIN,
-                   stream=None) -> PlaneImage:
+                   volume_engine: int = A.ENGINE_WAVEFRONT, stream=None) -> PlaneImage:
     sc = scenes.wedge_scene().upload(ctx)       # prism_on_plane.rs:91-93
     scr = scenes.plane_floor_screen(floor_side, floor_side)
     bm = scenes.plane_beam()
@@ -50,7 +50,8 @@ def prism_on_plane(ctx: S.Context, width: int = 800, height: int = 600, n_causti
         # volumetric pass, prism_on_plane.rs:157-164 (the beam estimator, z-buffered)
         hz = scenes.PLANE_HAZE
         vol = S.render_volumetric_scene(ctx, sc, cam, bm, width, height, n_volume, hz["sigma_s"], hz["sigma_t"],
-                                        hz["g"], hz["max_dist"], depth, 0xBEE5, engine=engine, stream=stream)
+                                        hz["g"], hz["max_dist"], depth, 0xBEE5, engine=volume_engine,
+                                        stream=stream)
         films.append(vol)
         # exposure and tonemap, prism_on_plane.rs:166-182
         p = S.display_quantile(ctx, vol, surface, 1, 1.0, 0.995)

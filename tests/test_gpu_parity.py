@@ -158,6 +158,26 @@ def test_legacy_and_beam_estimators(ctx, oracle):
         assert (err < FILM_TOL).all()
 
 
+@pytest.mark.parametrize("chunk_log2", [12, 22])
+def test_beam_estimator_wavefront_engine(ctx, oracle, chunk_log2):
+    # the beam estimator with its segments queued and their steps spread over
+    # waves: the oracle at the bar, the per-photon form to summation order;
+    # chunk_log2 = 12 splits 2000 photons into 4 chunks of 512
+    sc, bm, cam = scenes.dsotm_scene()
+    vp = A.VolParams(0.5, 0.06, 0.5, 14.0)
+    old = ctx.tuning()
+    ctx.set_tuning(chunk_log2=chunk_log2)
+    try:
+        g, c = _volume_pair(ctx, oracle, sc, cam, bm, 96, 96, 2_000, vp, 11, est=A.EST_BEAM,
+                            engine=A.ENGINE_WAVEFRONT)
+    finally:
+        ctx.set_tuning(chunk_log2=old.chunk_log2)
+    p, _ = _volume_pair(ctx, oracle, sc, cam, bm, 96, 96, 2_000, vp, 11, est=A.EST_BEAM)
+    err, err_p = rel_l1_per_channel(g, c), rel_l1_per_channel(g, p)
+    print("beam wavefront: rel L1 vs oracle", err, "vs per-photon", err_p)
+    assert (err < FILM_TOL).all() and (err_p < 1e-4).all() and c[..., 1].sum() > 0
+
+
 def test_chunked_equals_single_dispatch(ctx, oracle):
     # photon_base chunking (vol_photons.rs:590-619): two halves == one dispatch
     sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera(160, 90)
