@@ -6,10 +6,10 @@
 // segment.  Here the photons are traced first, each segment (origin,
 // direction, length, power, wavelength, step count) appended to a queue in
 // HBM; then persistent waves take one segment at a time and spread its
-// steps over their 64 lanes.  The RNG is drawn only by the trace (the beam
-// estimator draws none per step), and each step's splat is the same
-// function as in the per-photon form, so films differ only in the order of
-// their fp32 sums.
+// steps over their 64 lanes, summing overlapping splats in LDS first.  The
+// RNG is drawn only by the trace (the beam estimator draws none per step),
+// and each step's splat is the same function as in the per-photon form, so
+// films differ only in the order of their fp32 sums.
 #include "spx_internal.hpp"
 #include "spx_paths.hpp"
 
@@ -49,17 +49,55 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
     }
 }
 
-// Persistent waves: a wave claims the next queued segment and its lanes
-// take steps lane, lane + 64, ... of it; deposits are fp32 atomics into the
-// block's film copy.
-__global__ __launch_bounds__(256) void k_beam_deposit(PhotonParams P, const float4 *segs, const uint32_t *n_segs,
-                                                      uint32_t *cursor) {
+// A wave's LDS tile over the bounding box of 64 consecutive steps'
+// footprints: consecutive steps are half a pixel apart and their tents
+// overlap, so summing them in LDS first turns several film atomics per
+// pixel into one.  A deposit outside the box goes to the film directly.
+constexpr uint32_t kBeamTilePix = 1024;
+
+struct TileSink {
+    float *tile;
+    float *film;
+    int bx0, by0, bw, bh;
+};
+
+__device__ inline void deposit_xy(TileSink &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
+    const int x = (int)px - s.bx0, y = (int)py - s.by0;
+    if (x < 0 || x >= s.bw || y < 0 || y >= s.bh) {
+        film_add(s.film, idx, v);
+        return;
+    }
+    float *t = s.tile + 3 * (y * s.bw + x);
+    __hip_atomic_fetch_add(t + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(t + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(t + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ inline int wave_min(int v) {
+    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+    return v;
+}
+__device__ inline int wave_max(int v) {
+    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+    return v;
+}
+
+// Persistent one-wave blocks: a wave claims the next queued segment and
+// takes its steps 64 at a time, lane k0 + lane; each group of 64 splats
+// into the LDS tile over their footprints' bounding box (as in
+// splat_transverse, volume.rs:178-218), which is then added to the block's
+// film copy; groups whose box exceeds the tile splat straight to the film.
+__global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float4 *segs, const uint32_t *n_segs,
+                                                     uint32_t *cursor) {
     __shared__ float4 tbl[kTblN];
+    __shared__ float tile[3 * kBeamTilePix];
     for (uint32_t k = threadIdx.x; k < (uint32_t)kTblN; k += blockDim.x) tbl[k] = P.tbl[k];
     __syncthreads();
     const uint32_t total = *n_segs;
-    GlobalSink sink{deposit_film(P)};
-    const uint32_t lane = __lane_id();
+    float *film = deposit_film(P);
+    GlobalSink direct{film};
+    const uint32_t lane = threadIdx.x;
+    const int W = (int)P.w, H = (int)P.h;
     while (true) {
         uint32_t j = 0;
         if (lane == 0) j = atomicAdd(cursor, 1u);
@@ -71,7 +109,44 @@ __global__ __launch_bounds__(256) void k_beam_deposit(PhotonParams P, const floa
         const uint32_t m = __float_as_uint(c.y);
         const float ds = a.w / (float)m;
         const f3 cmf = cmf_lookup(tbl, c.x);
-        for (uint32_t k = lane; k < m; k += 64) beam_step(P, ro, rd, b.w, cmf, ds, k, sink);
+        for (uint32_t k0 = 0; k0 < m; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            BeamSplat sp;
+            const bool ok = k < m && beam_step_prep(P, ro, rd, b.w, cmf, ds, k, sp);
+            // the footprint splat_transverse walks (plus its rounded-centre fallback)
+            int x0 = 0x7FFFFFFF, x1 = -1, y0 = 0x7FFFFFFF, y1 = -1;
+            if (ok) {
+                const float cx = sp.s * (float)W - 0.5f, cy = (1.0f - sp.t) * (float)H - 0.5f;
+                const float r = fmaxf(sp.radius, 0.5f);
+                x0 = (int)fmaxf(floorf(cx - r), 0.0f);
+                x1 = (int)ceilf(cx + r);
+                x1 = x1 < 0 ? 0 : (x1 > W - 1 ? W - 1 : x1);
+                y0 = (int)fmaxf(floorf(cy - r), 0.0f);
+                y1 = (int)ceilf(cy + r);
+                y1 = y1 < 0 ? 0 : (y1 > H - 1 ? H - 1 : y1);
+            }
+            const int bx0 = wave_min(x0), bx1 = wave_max(x1), by0 = wave_min(y0), by1 = wave_max(y1);
+            if (bx1 < 0) continue;  // no step of the group is on the film
+            const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+            if (bw <= 0 || bh <= 0 || bw * bh > (int)kBeamTilePix) {
+                if (ok) splat_transverse(P, sp.s, sp.t, sp.color, sp.radius, direct);
+                continue;
+            }
+            const int area = bw * bh;
+            for (int e = (int)lane; e < 3 * area; e += 64) tile[e] = 0.0f;
+            __syncthreads();
+            if (ok) {
+                TileSink ts{tile, film, bx0, by0, bw, bh};
+                splat_transverse(P, sp.s, sp.t, sp.color, sp.radius, ts);
+            }
+            __syncthreads();
+            for (int e = (int)lane; e < area; e += 64) {
+                const f3 v = mk(tile[3 * e], tile[3 * e + 1], tile[3 * e + 2]);
+                if (v.x != 0.0f || v.y != 0.0f || v.z != 0.0f)
+                    film_add(film, (uint32_t)((by0 + e / bw) * W + bx0 + e % bw), v);
+            }
+            __syncthreads();
+        }
     }
 }
 
@@ -91,7 +166,7 @@ hipError_t run_beam_wavefront(spx_ctx *ctx, const PhotonParams &p, void *scratch
         hipLaunchKernelGGL(k_beam_trace, dim3((count + 255) / 256), dim3(256), 0, s, p, start, count, segs,
                            counters);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 8), dim3(256), 0, s, p, segs, counters, counters + 1);
+        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 10), dim3(64), 0, s, p, segs, counters, counters + 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

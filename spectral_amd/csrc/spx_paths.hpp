@@ -327,7 +327,7 @@ __device__ inline void splat_transverse(const PhotonParams &P, float s, float t,
         long px = (long)roundf(cx), py = (long)roundf(cy);
         px = px < 0 ? 0 : (px > (long)W - 1 ? (long)W - 1 : px);
         py = py < 0 ? 0 : (py > (long)H - 1 ? (long)H - 1 : py);
-        sink.add((uint32_t)(py * (long)W + px), color);
+        deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * (long)W + px), color);
         return;
     }
     float inv = 1.0f / wsum;
@@ -335,7 +335,7 @@ __device__ inline void splat_transverse(const PhotonParams &P, float s, float t,
         for (long px = x0; px <= x1; ++px) {
             float dx = (float)px - cx, dy = (float)py - cy;
             float w = fmaxf(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f) * inv;
-            if (w > 0.0f) sink.add((uint32_t)(py * (long)W + px), color * w);
+            if (w > 0.0f) deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * (long)W + px), color * w);
         }
 }
 
@@ -357,25 +357,31 @@ __device__ inline uint32_t beam_step_count(const PhotonParams &P, f3 ro, f3 rd, 
     return (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
 }
 
-template <class Sink>
-__device__ inline void beam_step(const PhotonParams &P, f3 ro, f3 rd, float power, f3 cmf, float ds, uint32_t k,
-                                 Sink &sink) {
+// Step k of a segment: its pixel-space centre, tent radius and colour
+// (volume.rs:277-305); false when the point is off the film or behind the
+// z-buffer.
+struct BeamSplat {
+    float s, t, radius;
+    f3 color;
+};
+
+__device__ inline bool beam_step_prep(const PhotonParams &P, f3 ro, f3 rd, float power, f3 cmf, float ds, uint32_t k,
+                                      BeamSplat &o) {
     const f3 cam_o = ld3(P.proj.o);
     float dist = ((float)k + 0.5f) * ds;
     f3 p = ro + rd * dist;
     float s, t;
-    if (!project(P.proj, p, s, t)) return;
+    if (!project(P.proj, p, s, t)) return false;
     uint32_t idx = pix_index(s, t, P.w, P.h);
     f3 to = cam_o - p;
     float d = fmaxf(length(to), 1e-3f);
     float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
-    if (!(d < zb)) return;
+    if (!(d < zb)) return false;
     f3 view = to / d;
     float cos_theta = dot(rd, view);
     float phase = phase_hg(P.vp.g, cos_theta);
     float trans = expf(-P.vp.sigma_t * d);
     float weight = power * P.vp.sigma_s * phase * trans / (d * d) * ds;
-    f3 color = cmf * weight;
     f3 perp = cross(view, mk(0.f, 1.f, 0.f));
     if (dot(perp, perp) < 1e-8f) perp = cross(view, mk(1.f, 0.f, 0.f));
     perp = normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
@@ -388,7 +394,18 @@ __device__ inline void beam_step(const PhotonParams &P, f3 ro, f3 rd, float powe
     } else {
         radius_px = 1.0f;
     }
-    splat_transverse(P, s, t, color, radius_px, sink);
+    o.s = s;
+    o.t = t;
+    o.radius = radius_px;
+    o.color = cmf * weight;
+    return true;
+}
+
+template <class Sink>
+__device__ inline void beam_step(const PhotonParams &P, f3 ro, f3 rd, float power, f3 cmf, float ds, uint32_t k,
+                                 Sink &sink) {
+    BeamSplat b;
+    if (beam_step_prep(P, ro, rd, power, cmf, ds, k, b)) splat_transverse(P, b.s, b.t, b.color, b.radius, sink);
 }
 
 template <class Sink>
