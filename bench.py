@@ -275,6 +275,9 @@ def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
     lanes = profile_figure("batched_step_mixed512_sq_counters.md", f"lanes_active[{name}]")
     if lanes is not None:
         out["lanes_active"] = lanes
+    if busy is not None and lanes is not None:
+        # the share of the SIMDs' lane slots doing useful VALU work
+        out["valu_lane_frac"] = round(busy * lanes / 64.0, 3)
     return out
 
 
@@ -421,8 +424,8 @@ def main() -> int:
     dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
     roofline = dict(groups[dominant])
     roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
-                                                 "frac", "traffic", "valu_busy", "lanes_active", "atomics_gops",
-                                                 "atomic_frac")
+                                                 "frac", "traffic", "valu_busy", "lanes_active", "valu_lane_frac",
+                                                 "atomics_gops", "atomic_frac")
                                if k in v}
                            for g, v in groups.items()}
     roofline["atomic_peak_gops"] = round(a_peak, 2)
