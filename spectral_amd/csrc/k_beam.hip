@@ -53,7 +53,7 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
 // footprints: consecutive steps are half a pixel apart and their tents
 // overlap, so summing them in LDS first turns several film atomics per
 // pixel into one.  A deposit outside the box goes to the film directly.
-constexpr uint32_t kBeamTilePix = 1024;
+constexpr uint32_t kBeamTilePix = 1536;
 
 struct TileSink {
     float *tile;
@@ -166,7 +166,7 @@ hipError_t run_beam_wavefront(spx_ctx *ctx, const PhotonParams &p, void *scratch
         hipLaunchKernelGGL(k_beam_trace, dim3((count + 255) / 256), dim3(256), 0, s, p, start, count, segs,
                            counters);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 10), dim3(64), 0, s, p, segs, counters, counters + 1);
+        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 8), dim3(64), 0, s, p, segs, counters, counters + 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;
