@@ -137,7 +137,7 @@ def cpu_baseline(paths_total, photons_total, target_s=12.0):
     ph_rate = nph / t_ph
     step_s = paths_total / cam_rate + photons_total / ph_rate
     work = {"seg_per_path": cst.segments / cst.paths, "seg_per_photon": segs / nph,
-            "splat_per_photon": splats / nph}
+            "splat_per_photon": splats / nph, "configs": config_work(O)}
     return {"value": round((paths_total + photons_total) / step_s / 1e6, 3), "unit": UNIT,
             "cores": cores, "kind": "port",
             "sample": (f"oracle (C restatement of spectral-core): rainbow 256x256x{spp} spp on {cores} "
@@ -145,6 +145,58 @@ def cpu_baseline(paths_total, photons_total, target_s=12.0):
                        f"(3 prism rotations) at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; "
                        f"value = the mixed step's paths and photons at those rates"),
             "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4)}, work
+
+
+def config_work(O):
+    """Segments and deposits per unit of each single-job configuration, from
+    the oracle's counters on small samples of the same scenes (the film
+    sizes shrunk where they only set the pixel grid)."""
+    from spectral_amd import _abi as A
+    from spectral_amd import scenes
+    from spectral_amd.core import opts
+    vp, n = A.VolParams(**scenes.HAZE), 1 << 14
+    out = {}
+    _, st = O.volume(scenes.prism_scene(), scenes.prism_camera(320, 180), scenes.prism_beam(), 320, 180, n, 0, vp,
+                     None, 0, A.EST_POINT, opts(A.ILLUM_D65), threads=4)
+    out["prism"] = {"seg": st.segments / st.paths, "splat": st.splats / st.paths}
+    _, st = O.camera_accumulate(scenes.rainbow_scene(), scenes.rainbow_camera(128, 128), 128, 128, 4, 0, 1,
+                                opts(A.ILLUM_D65, True, 8, 3), threads=4)
+    out["rainbow"] = {"seg": st.segments / st.paths}
+    fs, fc = scenes.prism_scene(floor=True), scenes.floor_camera(480, 270)
+    _, depth = O.surface_rim(fs, fc, 480, 270)
+    _, st = O.volume(fs, fc, scenes.prism_beam(), 480, 270, n, 0, vp, depth.ravel(), 0, A.EST_POINT,
+                     opts(A.ILLUM_D65), threads=4)
+    out["floor_volume"] = {"seg": st.segments / st.paths, "splat": st.splats / st.paths}
+    _, st = O.lighttrace(scenes.prism_scene(), scenes.floor_screen(1024, 1024), scenes.prism_beam(), n, 0, 0x0F1,
+                         opts(A.ILLUM_D65), threads=4)
+    out["floor_caustic"] = {"seg": st.segments / st.paths, "splat": st.splats / st.paths}
+    return {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in out.items()}
+
+
+def config_roofline(out, work, a_peak):
+    """Per-configuration roofline: algorithmic HBM bytes (the
+    wavefront state model) and deposit atomics over each measured rate,
+    against 8 TB/s and the measured atomic peak."""
+    rows = {}
+    for key, rate in out.items():
+        if not key.endswith(("_mphotons_s", "_mpaths_s")):
+            continue
+        cls = ("rainbow" if key.startswith("rainbow") else "prism" if key.startswith("prism_2") else
+               "floor_volume" if key.startswith("floor_volume") else
+               "floor_caustic" if key.startswith("floor_caustic") else None)
+        if cls is None or cls not in work:
+            continue
+        wm = work[cls]
+        units = rate * 1e6  # per second
+        if cls == "rainbow":
+            bytes_s = units * (wm["seg"] * CAM_SEG_BYTES + CAM_PATH_BYTES)
+            atom_s = units * 3
+        else:
+            bytes_s = units * (wm["seg"] * PH_SEG_BYTES + wm["splat"] * SPLAT_BYTES)
+            atom_s = units * wm["splat"] * 3
+        rows[key[:key.rfind("_m")]] = {"hbm_frac": round(bytes_s / 1e9 / HBM_PEAK_GBS, 4),
+                                       "atomic_frac": round(atom_s / 1e9 / a_peak, 4) if a_peak else None}
+    return rows
 
 
 def per_config(ctx, timer, quick):
@@ -317,8 +369,9 @@ def main() -> int:
                     help="only run a 256 MiB device copy 4x (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the light-transport jobs are independent and this "
-              "build runs no multi-GPU path (see DESIGN.md for how it would shard)", flush=True)
+        print("bench.py measures one GPU only: the light-transport jobs are independent (they would "
+              "shard by job, or by photon / sample ranges summed with one film reduce) and no "
+              "multi-GPU path is run here", flush=True)
         return 2
 
     import spectral_amd as S
@@ -412,8 +465,8 @@ def main() -> int:
     }
     if kt["bins"][1]:
         groups["bins"] = kernel_roofline("k_bins passes", dep_bytes, kt["bins"][0] / steps, kt["bins"][1] // steps)
-    # SURVEY 8(d)'s second ceiling: the deposits as fp32 atomics (3 per
-    # deposit) over the photon kernel's time, against the measured A_peak.
+    # The second ceiling: the deposits as fp32 atomics (3 per deposit) over
+    # the photon kernel's time, against the measured atomic peak.
     a_peak = atomic_peak(ctx, timer)
     ph_ms = groups["photon"]["kernel_ms_per_launch"] * groups["photon"]["launches_per_step"]
     dep_atomics = photons * wm["splat_per_photon"] * 3
@@ -433,7 +486,9 @@ def main() -> int:
                         "persistent kernels keep path state in registers, so measured HBM traffic is the "
                         "smaller `traffic` and the bound is VALU issue (`valu_busy`), with `lanes_active` of a "
                         "wave's 64 lanes working on average")
-    roofline["work_model"] = {k: round(v, 4) for k, v in wm.items()}
+    roofline["work_model"] = {k: round(v, 4) for k, v in wm.items() if k != "configs"}
+    if "configs" in wm:
+        roofline["work_model_configs"] = wm["configs"]
     roofline["work_model_source"] = "oracle counts on the cpu_baseline samples" if cpu else "pinned constants"
     result = {
         "metric": METRIC, "value": round(value, 2), "unit": UNIT, "n_gpus": 1,
@@ -452,6 +507,8 @@ def main() -> int:
     }
     if not args.no_configs:
         result["configs"] = per_config(ctx, timer, args.quick)
+        if "configs" in wm:
+            result["configs_roofline"] = config_roofline(result["configs"], wm["configs"], a_peak)
         log(json.dumps(result["configs"]))
     if not args.no_serve:
         from spectral_amd import serve
