@@ -406,6 +406,11 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
         if (e == hipSuccess) e = hipEventRecord(resolved, B.s2);
         ++k;
     }
+    // The concurrent camera round's tail: once the photon rounds are traced,
+    // a full grid joins its blocks in draining the camera queue (blocks that
+    // find it empty exit at once), so the step does not wait on a camera
+    // round left with its small grid alone.
+    if (e == hipSuccess && S.cam_blocks) e = trace(0, B.bst[0]);
     if (e == hipSuccess && S.overlap && k) e = hipStreamWaitEvent(s, B.ev[2 * (k - 1) + 1], 0);  // join
     if (e == hipSuccess && S.cam_blocks) e = hipStreamWaitEvent(s, B.cev[1], 0);
     if (e == hipSuccess && !S.reps.empty())
