@@ -274,8 +274,9 @@ int spx_resolve_srgb8(spx_ctx *ctx, const float *film_sum, uint32_t width, uint3
  * max(kradius / 2, 0.5), times target_n / max(total_photons, 1); plus the
  * surface layer (3*w*h XYZ, NULL = none); times exposure; XYZ -> linear
  * sRGB, Reinhard c/(1+c), clamp below at 0, gamma 1/2.2.  rgba (4*w*h
- * bytes) receives round(255 * clamp(v, 0, 1)) per channel (the Rgba8Unorm
- * store) and alpha 255.  All buffers are device memory; row 0 = top. */
+ * bytes, 4-byte aligned) receives round(255 * clamp(v, 0, 1)) per channel
+ * (the Rgba8Unorm store) and alpha 255.  All buffers are device memory;
+ * row 0 = top. */
 int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
                         uint32_t height, uint32_t total_photons, float target_n, float exposure,
                         float kradius, uint8_t *rgba, void *stream);
