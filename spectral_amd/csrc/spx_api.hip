@@ -849,7 +849,8 @@ int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface
     if (!ctx) return SPX_EINVAL;
     int rc;
     if ((rc = check_film(ctx, vol_sum, width, height, "spx_display_resolve"))) return rc;
-    if (!rgba || ((uintptr_t)rgba & 3u)) return set_error(ctx, SPX_EINVAL, "spx_display_resolve: null or unaligned output");
+    if (!rgba || ((uintptr_t)rgba & 3u))
+        return set_error(ctx, SPX_EINVAL, "spx_display_resolve: null or unaligned output");
     if (!std::isfinite(target_n) || !std::isfinite(exposure) || !(kradius >= 0.0f) || !(kradius <= 32.0f))
         return set_error(ctx, SPX_EINVAL, "spx_display_resolve: bad target_n, exposure or kradius (0..32)");
     const float sigma = fmaxf(kradius * 0.5f, 0.5f);
