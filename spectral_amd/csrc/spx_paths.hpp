@@ -304,39 +304,65 @@ __device__ inline void volume_point_segment(const PhotonParams &P, const float4 
 }
 
 // Beam estimator deposits for one segment (volume.rs:263-311, 178-218).
+//
+// A texel with dx^2 + dy^2 >= r^2 has weight max(1 - sqrt(d2) / r, 0) = 0
+// exactly (sqrt and the division round monotonically and r / r = 1), adds
+// +0 to the weight sum and is not deposited, so each row walks only the
+// span where d2 < r^2 can hold (widened by a pixel against rounding in the
+// span itself) and an exact test, fma(-r, r, d2) >= 0, skips the rest: the
+// same sums and deposits, in the same row-major order, as walking the whole
+// box.  Coordinates fit in 32 bits (the box is clamped to the film).
+__device__ inline void tent_row_span(float cx, float r, float dy, int x0, int x1, int &xs, int &xe) {
+    const float hw = sqrtf(fmaxf(fmaf(-dy, dy, r * r), 0.0f)) + 1.0f;
+    xs = max(x0, (int)floorf(cx - hw));
+    xe = min(x1, (int)ceilf(cx + hw));
+}
+
 template <class Sink>
 __device__ inline void splat_transverse(const PhotonParams &P, float s, float t, f3 color,
                                         float radius_px, Sink &sink) {
-    const uint32_t W = P.w, H = P.h;
-    float cx = s * (float)W - 0.5f;
-    float cy = (1.0f - t) * (float)H - 0.5f;
-    float r = fmaxf(radius_px, 0.5f);
-    long x0 = (long)fmaxf(floorf(cx - r), 0.0f);
-    long x1 = (long)ceilf(cx + r);
-    x1 = x1 < 0 ? 0 : (x1 > (long)W - 1 ? (long)W - 1 : x1);
-    long y0 = (long)fmaxf(floorf(cy - r), 0.0f);
-    long y1 = (long)ceilf(cy + r);
-    y1 = y1 < 0 ? 0 : (y1 > (long)H - 1 ? (long)H - 1 : y1);
+    const int W = (int)P.w, H = (int)P.h;
+    const float cx = s * (float)W - 0.5f;
+    const float cy = (1.0f - t) * (float)H - 0.5f;
+    const float r = fmaxf(radius_px, 0.5f);
+    const int x0 = (int)fmaxf(floorf(cx - r), 0.0f);
+    int x1 = (int)ceilf(cx + r);
+    x1 = x1 < 0 ? 0 : (x1 > W - 1 ? W - 1 : x1);
+    const int y0 = (int)fmaxf(floorf(cy - r), 0.0f);
+    int y1 = (int)ceilf(cy + r);
+    y1 = y1 < 0 ? 0 : (y1 > H - 1 ? H - 1 : y1);
     float wsum = 0.0f;
-    for (long py = y0; py <= y1; ++py)
-        for (long px = x0; px <= x1; ++px) {
-            float dx = (float)px - cx, dy = (float)py - cy;
-            wsum += fmaxf(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f);
+    for (int py = y0; py <= y1; ++py) {
+        const float dy = (float)py - cy;
+        int xs, xe;
+        tent_row_span(cx, r, dy, x0, x1, xs, xe);
+        for (int px = xs; px <= xe; ++px) {
+            const float dx = (float)px - cx;
+            const float d2 = dx * dx + dy * dy;
+            if (fmaf(-r, r, d2) >= 0.0f) continue;
+            wsum += fmaxf(1.0f - sqrtf(d2) / r, 0.0f);
         }
+    }
     if (wsum <= 0.0f) {
-        long px = (long)roundf(cx), py = (long)roundf(cy);
-        px = px < 0 ? 0 : (px > (long)W - 1 ? (long)W - 1 : px);
-        py = py < 0 ? 0 : (py > (long)H - 1 ? (long)H - 1 : py);
-        deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * (long)W + px), color);
+        int px = (int)roundf(cx), py = (int)roundf(cy);
+        px = px < 0 ? 0 : (px > W - 1 ? W - 1 : px);
+        py = py < 0 ? 0 : (py > H - 1 ? H - 1 : py);
+        deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * W + px), color);
         return;
     }
-    float inv = 1.0f / wsum;
-    for (long py = y0; py <= y1; ++py)
-        for (long px = x0; px <= x1; ++px) {
-            float dx = (float)px - cx, dy = (float)py - cy;
-            float w = fmaxf(1.0f - sqrtf(dx * dx + dy * dy) / r, 0.0f) * inv;
-            if (w > 0.0f) deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * (long)W + px), color * w);
+    const float inv = 1.0f / wsum;
+    for (int py = y0; py <= y1; ++py) {
+        const float dy = (float)py - cy;
+        int xs, xe;
+        tent_row_span(cx, r, dy, x0, x1, xs, xe);
+        for (int px = xs; px <= xe; ++px) {
+            const float dx = (float)px - cx;
+            const float d2 = dx * dx + dy * dy;
+            if (fmaf(-r, r, d2) >= 0.0f) continue;
+            const float w = fmaxf(1.0f - sqrtf(d2) / r, 0.0f) * inv;
+            if (w > 0.0f) deposit_xy(sink, (uint32_t)px, (uint32_t)py, (uint32_t)(py * W + px), color * w);
         }
+    }
 }
 
 // Beam estimator, split so that a segment's steps can also be spread over

@@ -156,6 +156,11 @@ def test_legacy_and_beam_estimators(ctx, oracle):
         err = rel_l1_per_channel(g, c)
         print("estimator", est, "rel L1", err)
         assert (err < FILM_TOL).all()
+        if est == A.EST_BEAM:
+            # every texel of every tent, in the same order: the films differ
+            # only in the order of fp32 atomic sums (the oracle with each
+            # tent's last column left out of its deposits reads 5.5e-3)
+            assert (err < 2e-5).all()
 
 
 @pytest.mark.parametrize("chunk_log2", [12, 22])
@@ -175,7 +180,7 @@ def test_beam_estimator_wavefront_engine(ctx, oracle, chunk_log2):
     p, _ = _volume_pair(ctx, oracle, sc, cam, bm, 96, 96, 2_000, vp, 11, est=A.EST_BEAM)
     err, err_p = rel_l1_per_channel(g, c), rel_l1_per_channel(g, p)
     print("beam wavefront: rel L1 vs oracle", err, "vs per-photon", err_p)
-    assert (err < FILM_TOL).all() and (err_p < 1e-4).all() and c[..., 1].sum() > 0
+    assert (err < 2e-5).all() and (err_p < 1e-4).all() and c[..., 1].sum() > 0
 
 
 def test_chunked_equals_single_dispatch(ctx, oracle):
