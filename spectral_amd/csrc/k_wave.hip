@@ -220,6 +220,7 @@ __device__ inline void ph_load(const PhQueue &q, uint32_t i, PhotonState &st) {
     st.rng = q.rng[i];
     st.lambda = q.lam[i];
     st.power = q.pw[i];
+    st.n_glass = kNoGlass;
 }
 
 struct PhWave {

@@ -79,6 +79,9 @@ __device__ inline float hero_lambda(float u, int j) {
 // ---------------------------------------------------------------------------
 // Glass::n, sellmeier.rs:28-44 -- fp64 inside, fp32 out.
 // ---------------------------------------------------------------------------
+// No glass: scene creation admits glass ids 0..2 only.
+constexpr uint32_t kNoGlass = 0xFFFFFFFFu;
+
 __device__ inline float glass_n(uint32_t glass, float lambda_nm) {
     double l_um = (double)lambda_nm / 1000.0;
     if (glass == SPX_GLASS_WATER) return (float)(1.3238 + 0.00314 / (l_um * l_um));

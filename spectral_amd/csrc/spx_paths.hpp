@@ -121,6 +121,8 @@ struct PhotonState {
     uint32_t rng;
     float lambda;
     float power;
+    float n;           // Glass::n of glass n_glass at this photon's wavelength
+    uint32_t n_glass;  // kNoGlass until the first dielectric hit
 };
 
 __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, uint32_t i,
@@ -132,6 +134,7 @@ __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, ui
     st.lambda = kLambdaMin + kLambdaRange * next_f32(st.rng);
     st.rd = ld3(P.beam.dir);
     st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
+    st.n_glass = kNoGlass;
 }
 
 // A deposit at film pixel (px, py) = index idx; sinks keyed by film tile
@@ -148,11 +151,24 @@ __device__ inline void sink_photon_begin(Sink &) {}
 template <class Sink>
 __device__ inline void sink_scattered(Sink &, uint32_t) {}
 
+// material_n at the photon's wavelength.  The wavelength is fixed for the
+// photon's life, so Glass::n (fp64 Sellmeier: the costliest step of a
+// dielectric hit) is computed at its first hit of a glass and reused, the
+// same value, at later hits of that glass.
+__device__ inline float photon_n(const spx_material &m, const PhotonParams &P, PhotonState &st) {
+    if (m.kind != SPX_MAT_DIELECTRIC) return 1.0f;
+    if (st.n_glass != m.glass) {
+        st.n = material_n(m, st.lambda, P.dispersion);
+        st.n_glass = m.glass;
+    }
+    return st.n;
+}
+
 // Scatter at a hit (the tail of every photon loop).  Returns tir.
 __device__ inline bool photon_scatter(const SceneData &S, const PhotonParams &P, int mi, const Hit &h,
                                       PhotonState &st) {
     const spx_material &m = S.mats[mi];
-    float nh = material_n(m, st.lambda, P.dispersion);
+    float nh = photon_n(m, P, st);
     f3 nd;
     bool tir;
     float w = scatter(m, st.rd, h, nh, st.rng, nd, tir);
