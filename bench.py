@@ -144,7 +144,22 @@ def cpu_baseline(paths_total, photons_total, target_s=12.0):
                        f"threads = {cam_rate / 1e6:.3f} Mpaths/s; prism-on-floor volume 3 x {n} photons "
                        f"(3 prism rotations) at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; "
                        f"value = the mixed step's paths and photons at those rates"),
-            "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4)}, work
+            "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4),
+            "host": host_cpu()}, work
+
+
+def host_cpu():
+    """The host the CPU baseline ran on: logical CPUs and the CPU model."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu": model}
 
 
 def config_work(O):
