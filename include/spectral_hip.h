@@ -333,8 +333,8 @@ int spx_glass_n(spx_ctx *ctx, uint32_t glass, const float *lambda_host, uint32_t
 
 typedef struct {
     uint32_t kind;        /* SPX_JOB_* */
-    uint32_t width;       /* film (camera, volume) or screen (lighttrace) size */
-    uint32_t height;
+    uint32_t width;       /* film size (camera, volume); a lighttrace job's */
+    uint32_t height;      /* film is its screen's width x height */
     uint32_t spp;         /* camera */
     uint32_t sample_base; /* camera RNG counter base */
     uint32_t n_photons;   /* volume, lighttrace */

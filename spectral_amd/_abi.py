@@ -99,9 +99,9 @@ class Tuning(C.Structure):
 
 
 class BatchCfg(C.Structure):
+    """spx_batch_cfg: per-call overrides of the context's tuning (0 = tuned)."""
     _fields_ = [("splat_mode", C.c_uint32), ("blocks", C.c_uint32), ("chunk_paths", C.c_uint32),
-                ("chunk_photons", C.c_uint32), ("overlap_bins", C.c_uint32),
-                ("camera_blocks", C.c_uint32)]
+                ("chunk_photons", C.c_uint32)]
 
 
 class Request(C.Structure):

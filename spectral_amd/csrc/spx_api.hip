@@ -366,6 +366,18 @@ int spx::photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *bea
     return SPX_OK;
 }
 
+// A screen whose rectangle is finite and spans an area (Screen::new,
+// lighttrace.rs:27-34, divides by |u|^2 and |v|^2 and normalises u x v).
+int spx::check_screen(spx_ctx *ctx, const spx_screen &scr, const char *who) {
+    if (!finite3(scr.corner) || !finite3(scr.u) || !finite3(scr.v))
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": non-finite screen");
+    PhotonParams p{};
+    set_screen(p, scr);
+    if (!(p.scr_uu > 0.0f) || !(p.scr_vv > 0.0f) || !finite3(p.scr_n))
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": degenerate screen rectangle");
+    return SPX_OK;
+}
+
 void spx::set_screen(PhotonParams &p, const spx_screen &scr) {
     // Screen::new, lighttrace.rs:27-34
     f3 u = ld3(scr.u), v = ld3(scr.v), n = normalize(cross(u, v));
@@ -658,6 +670,7 @@ int spx_lighttrace(spx_ctx *ctx, const spx_scene *scene, const spx_screen *scree
     if (rc) return rc;
     if (!screen) return set_error(ctx, SPX_EINVAL, "spx_lighttrace: null screen");
     if ((rc = check_film(ctx, screen_sum, screen->width, screen->height, "spx_lighttrace"))) return rc;
+    if ((rc = check_screen(ctx, *screen, "spx_lighttrace"))) return rc;
     PhotonParams p;
     if ((rc = photon_params(ctx, scene, beam, opts, n, photon_base, seed, p, "spx_lighttrace"))) return rc;
     set_screen(p, *screen);

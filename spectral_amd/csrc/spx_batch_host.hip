@@ -76,6 +76,7 @@ int build_batch_job(spx_ctx *ctx, const spx_job &jb, BatchJob &out, uint32_t idx
         return rc;
     if (jb.kind == SPX_JOB_LIGHTTRACE) {
         if ((rc = check_film(ctx, jb.film, jb.screen.width, jb.screen.height, who))) return rc;
+        if ((rc = check_screen(ctx, jb.screen, who))) return rc;
         set_screen(p, jb.screen);
     } else {
         if ((rc = check_film(ctx, jb.film, jb.width, jb.height, who))) return rc;

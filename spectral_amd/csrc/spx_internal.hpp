@@ -221,6 +221,7 @@ bool valid_opts(const spx_opts *o, bool camera);
 spx_opts default_opts(bool camera);
 bool finite3(const float *v);
 int check_film(spx_ctx *ctx, const float *film, uint32_t w, uint32_t h, const char *who);
+int check_screen(spx_ctx *ctx, const spx_screen &scr, const char *who);
 int photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *beam, const spx_opts *opts, uint32_t n,
                   uint32_t photon_base, uint32_t seed, PhotonParams &p, const char *who);
 void set_screen(PhotonParams &p, const spx_screen &scr);

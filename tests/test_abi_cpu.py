@@ -74,3 +74,22 @@ def test_tuning_layout_matches_header():
     body = body[:body.index("} spx_tuning;")]
     fields = re.findall(r"uint32_t (\w+);", body)
     assert fields == [n for n, _ in A.Tuning._fields_]
+
+
+def _header_fields(struct_name):
+    """(type, name) of each field of a typedef struct in the public header."""
+    text = (ROOT / "include" / "spectral_hip.h").read_text()
+    end = text.index("} " + struct_name + ";")
+    body = text[text.rindex("typedef struct {", 0, end) + len("typedef struct {"):end]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    return re.findall(r"([\w ]+?\**)\s*\b(\w+)(?:\[\d+\])?;", body)
+
+
+def test_batch_structs_match_header():
+    # the ctypes mirrors of spx_batch_cfg and spx_job: same fields, in order
+    for name, cls in (("spx_batch_cfg", A.BatchCfg), ("spx_job", A.Job)):
+        names = [n for _, n in _header_fields(name)]
+        assert names == [n for n, _ in cls._fields_], name
+    # spx_job: 10 u32 + opts 16 + camera 48 + beam 48 + screen 44 + vol 16 + 3 pointers
+    assert C.sizeof(A.Job) == 40 + 16 + 48 + 48 + 44 + 16 + 4 + 3 * 8
+    assert C.sizeof(A.BatchCfg) == 16

@@ -1,11 +1,15 @@
 """The batched engine: one persistent launch over a device job table gives
 each job the same film as running it alone (one launch per job), and the
 films agree with the CPU oracle."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
+import spectral_amd as S
 from spectral_amd import _abi as A
-from spectral_amd import jobs
+from spectral_amd import jobs, scenes
+from spectral_amd.core import opts
 
 from _metrics import rel_l1_per_channel
 
@@ -136,4 +140,58 @@ def test_batch_rejects_bad_jobs(ctx):
         br.run_batched()
         assert br.films[0].image().sum() > 0
     finally:
+        br.close()
+
+
+@pytest.mark.parametrize("splat_mode", [1, 2, 3])
+def test_lighttrace_jobs_in_a_batch(ctx, oracle, splat_mode):
+    # SPX_JOB_LIGHTTRACE entries (lighttrace::trace onto the floor screen,
+    # prism_on_plane.rs:107 pattern, two screen sizes) in one table with a
+    # camera and a volume job: every film equals its job run alone within
+    # summation order, and the screens match the oracle at the film bar
+    specs = [jobs.JobSpec(A.JOB_CAMERA, 64, spp=4, seed=1),
+             jobs.JobSpec(A.JOB_VOLUME, 96, n_photons=30000, seed=2, rotation=15.0)]
+    br = jobs.BatchRunner(ctx, specs)
+    sc, bm = scenes.prism_scene(), scenes.prism_beam()
+    ds = sc.upload(ctx)
+    lts = [(scenes.floor_screen(256, 256), 1 << 17, 0x0F1), (scenes.floor_screen(160, 96), 50_000, 7)]
+    films, table = [], list(br.table)
+    for scr, n, seed in lts:
+        f = S.Film(ctx, scr.width, scr.height)
+        films.append(f)
+        jb = A.Job()
+        jb.kind, jb.n_photons, jb.seed = A.JOB_LIGHTTRACE, n, seed
+        jb.opts = opts(A.ILLUM_D65)
+        jb.beam, jb.screen = bm, scr
+        jb.scene, jb.film = ds.handle.value, f.ptr
+        table.append(jb)
+    tbl = (A.Job * len(table))(*table)
+    try:
+        br.clear()
+        for f in films:
+            f.clear(0.0)
+        rc = A.lib().spx_batch_run(ctx.handle, tbl, len(table), C.byref(A.BatchCfg(splat_mode, 0, 0, 2048)), None)
+        assert rc == A.SPX_OK, A.lib().spx_last_error(ctx.handle)
+        mix = [f.image() for f in br.films] + [f.image() for f in films]
+        br.clear()
+        br.run_plain()
+        alone = [f.image() for f in br.films]
+        for (scr, n, seed) in lts:
+            alone.append(S.lighttrace_trace(ctx, ds, scr, bm, n, seed, illuminant=A.ILLUM_D65).image())
+        for j, (a, b) in enumerate(zip(mix, alone)):
+            assert a.sum() > 0, f"job {j} empty"
+            assert (rel_l1_per_channel(a, b) < ALONE_TOL).all(), f"job {j}"
+        for (scr, n, seed), g in zip(lts, mix[2:]):
+            c, _ = oracle.lighttrace(sc, scr, bm, n, 0, seed, opts(A.ILLUM_D65), threads=8)
+            err = rel_l1_per_channel(g, c)
+            print("batched lighttrace", scr.width, scr.height, "rel L1", err)
+            assert (err < 1e-6).all()  # summation order only (5e-8 measured)
+        # a degenerate screen is rejected before anything runs
+        bad = A.Job.from_buffer_copy(table[-1])
+        bad.screen.u[0] = bad.screen.u[1] = bad.screen.u[2] = 0.0
+        rc = A.lib().spx_batch_run(ctx.handle, (A.Job * 1)(bad), 1, None, None)
+        assert rc == A.SPX_EINVAL and b"degenerate screen" in A.lib().spx_last_error(ctx.handle)
+    finally:
+        for f in films:
+            f.free()
         br.close()
