@@ -214,13 +214,14 @@ __device__ inline void ph_store(const PhQueue &q, uint32_t i, const PhotonState 
     q.pw[i] = st.power;
 }
 
-__device__ inline void ph_load(const PhQueue &q, uint32_t i, PhotonState &st) {
+__device__ inline void ph_load(const PhQueue &q, uint32_t i, const float4 *tbl, PhotonState &st) {
     st.ro = mk(q.ox[i], q.oy[i], q.oz[i]);
     st.rd = mk(q.dx[i], q.dy[i], q.dz[i]);
     st.rng = q.rng[i];
     st.lambda = q.lam[i];
     st.power = q.pw[i];
     st.n_glass = kNoGlass;
+    st.cmf = cmf_lookup(tbl, st.lambda);
 }
 
 struct PhWave {
@@ -280,7 +281,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
             if (i < n_in) {
                 PhotonState st;
                 if (first) photon_begin(P, tbl, P.photon_base + start + i, st);
-                else ph_load(qin, i, st);
+                else ph_load(qin, i, tbl, st);
                 bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
                 uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
                 if (alive) ph_store(qout, at, st);

@@ -123,6 +123,7 @@ struct PhotonState {
     float power;
     float n;           // Glass::n of glass n_glass at this photon's wavelength
     uint32_t n_glass;  // kNoGlass until the first dielectric hit
+    f3 cmf;            // the CIE matching functions at this photon's wavelength
 };
 
 __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, uint32_t i,
@@ -135,6 +136,7 @@ __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, ui
     st.rd = ld3(P.beam.dir);
     st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
     st.n_glass = kNoGlass;
+    st.cmf = cmf_lookup(tbl, st.lambda);  // the wavelength is fixed: looked up once
 }
 
 // A deposit at film pixel (px, py) = index idx; sinks keyed by film tile
@@ -200,9 +202,7 @@ __device__ inline bool screen_coords(const PhotonParams &P, f3 o, f3 d, float &s
 
 // Screen::splat (texel_bilinear + 4-texel deposit), lighttrace.rs:65-91.
 template <class Sink>
-__device__ inline void screen_splat(const PhotonParams &P, const float4 *tbl, float s, float r,
-                                    float lambda, float power, Sink &sink) {
-    f3 c = cmf_lookup(tbl, lambda);
+__device__ inline void screen_splat(const PhotonParams &P, float s, float r, f3 c, float power, Sink &sink) {
     float W = (float)P.scr_w, H = (float)P.scr_h;
     float fx = s * W - 0.5f;
     fx = fx < 0.0f ? 0.0f : (fx > W - 1.0f ? W - 1.0f : fx);
@@ -254,7 +254,7 @@ __device__ inline int lighttrace_bounce(const SceneData &S, const float4 *tbl, c
     float scene_t = mi >= 0 ? h.t : INFINITY;
     float screen_t = on ? stt : INFINITY;
     if (screen_t < scene_t) {
-        screen_splat(P, tbl, ss, rr, st.lambda, st.power, sink);
+        screen_splat(P, ss, rr, st.cmf, st.power, sink);
         return 1;
     }
     if (mi < 0) return 2;
@@ -466,7 +466,7 @@ __device__ inline bool volume_bounce(const SceneData &S, const float4 *tbl, cons
     Hit h;
     int mi = scene_hit(S, st.ro, st.rd, h);
     float seg_len = mi >= 0 ? h.t : P.vp.max_dist;
-    f3 cmf = cmf_lookup(tbl, st.lambda);
+    const f3 cmf = st.cmf;
     if (EST == SPX_EST_POINT) volume_point_segment<false>(P, tbl, st, cmf, seg_len, st.rng, sink);
     else if (EST == SPX_EST_LEGACY) volume_point_segment<true>(P, tbl, st, cmf, seg_len, st.rng, sink);
     else volume_beam_segment(P, st, cmf, seg_len, sink);
