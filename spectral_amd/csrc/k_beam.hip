@@ -52,25 +52,29 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
 // A wave's LDS tile over the bounding box of 64 consecutive steps'
 // footprints: consecutive steps are half a pixel apart and their tents
 // overlap, so summing them in LDS first turns several film atomics per
-// pixel into one.  A deposit outside the box goes to the film directly.
-constexpr uint32_t kBeamTilePix = 1536;
+// pixel into one.  The 64 steps lie on one segment, so their colours share
+// its wavelength: each is cmf * weight, and the tile sums the scalar
+// weight * tent weight, multiplied by cmf once per pixel when it is
+// flushed (the same terms, reassociated: one LDS add per texel instead of
+// three).  A deposit outside the box goes to the film directly.
+constexpr uint32_t kBeamTilePix = 3072;  // 12 KiB: no group of the product image overflows it
 
 struct TileSink {
     float *tile;
     float *film;
+    f3 cmf;
     int bx0, by0, bw, bh;
 };
 
+// v = (weight * tent weight) in every channel (splat_transverse's colour
+// times the texel's weight, with colour = the step's scalar weight).
 __device__ inline void deposit_xy(TileSink &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
     const int x = (int)px - s.bx0, y = (int)py - s.by0;
     if (x < 0 || x >= s.bw || y < 0 || y >= s.bh) {
-        film_add(s.film, idx, v);
+        film_add(s.film, idx, s.cmf * v.x);
         return;
     }
-    float *t = s.tile + 3 * (y * s.bw + x);
-    __hip_atomic_fetch_add(t + 0, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(t + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    __hip_atomic_fetch_add(t + 2, v.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(s.tile + y * s.bw + x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ inline int wave_min(int v) {
@@ -90,7 +94,7 @@ __device__ inline int wave_max(int v) {
 __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float4 *segs, const uint32_t *n_segs,
                                                      uint32_t *cursor) {
     __shared__ float4 tbl[kTblN];
-    __shared__ float tile[3 * kBeamTilePix];
+    __shared__ float tile[kBeamTilePix];
     for (uint32_t k = threadIdx.x; k < (uint32_t)kTblN; k += blockDim.x) tbl[k] = P.tbl[k];
     __syncthreads();
     const uint32_t total = *n_segs;
@@ -133,17 +137,16 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                 continue;
             }
             const int area = bw * bh;
-            for (int e = (int)lane; e < 3 * area; e += 64) tile[e] = 0.0f;
+            for (int e = (int)lane; e < area; e += 64) tile[e] = 0.0f;
             __syncthreads();
             if (ok) {
-                TileSink ts{tile, film, bx0, by0, bw, bh};
-                splat_transverse(P, sp.s, sp.t, sp.color, sp.radius, ts);
+                TileSink ts{tile, film, cmf, bx0, by0, bw, bh};
+                splat_transverse(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight), sp.radius, ts);
             }
             __syncthreads();
             for (int e = (int)lane; e < area; e += 64) {
-                const f3 v = mk(tile[3 * e], tile[3 * e + 1], tile[3 * e + 2]);
-                if (v.x != 0.0f || v.y != 0.0f || v.z != 0.0f)
-                    film_add(film, (uint32_t)((by0 + e / bw) * W + bx0 + e % bw), v);
+                const float v = tile[e];
+                if (v != 0.0f) film_add(film, (uint32_t)((by0 + e / bw) * W + bx0 + e % bw), cmf * v);
             }
             __syncthreads();
         }
@@ -166,7 +169,9 @@ hipError_t run_beam_wavefront(spx_ctx *ctx, const PhotonParams &p, void *scratch
         hipLaunchKernelGGL(k_beam_trace, dim3((count + 255) / 256), dim3(256), 0, s, p, start, count, segs,
                            counters);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 8), dim3(64), 0, s, p, segs, counters, counters + 1);
+        // one-wave blocks of 13 KiB LDS (tile + CIE table): twelve resident
+        // per compute unit
+        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 12), dim3(64), 0, s, p, segs, counters, counters + 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

@@ -404,6 +404,7 @@ __device__ inline uint32_t beam_step_count(const PhotonParams &P, f3 ro, f3 rd, 
 // z-buffer.
 struct BeamSplat {
     float s, t, radius;
+    float weight;  // the step's scalar weight; color = cmf * weight
     f3 color;
 };
 
@@ -439,6 +440,7 @@ __device__ inline bool beam_step_prep(const PhotonParams &P, f3 ro, f3 rd, float
     o.s = s;
     o.t = t;
     o.radius = radius_px;
+    o.weight = weight;
     o.color = cmf * weight;
     return true;
 }
