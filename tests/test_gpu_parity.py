@@ -358,3 +358,22 @@ def test_binned_fixed_point_scaling_and_determinism(ctx):
                 assert np.array_equal(img, base * 2.0 ** k), k
     finally:
         ctx.set_tuning(splat_mode=old.splat_mode, bin_log2=old.bin_log2)
+
+
+def test_beam_wavefront_groups_larger_than_the_tile(ctx, oracle):
+    # a camera 0.15-1.35 units above the DSOTM beam sheet, looking down on
+    # it: tent boxes of 11-50 pixels a side (4 % of the steps 45 or more, at
+    # the 24 px clamp; the oracle's counts), so the boxes of the groups of 64
+    # steps with the largest tents exceed the LDS tile (3072 pixels) and
+    # those groups splat straight to the film; the film still equals the
+    # per-photon form to summation order and the oracle
+    sc, bm, _ = scenes.dsotm_scene()
+    cam = S.Camera.look_at((-1.93, -0.52, 0.75), (-1.93, -0.52, 0.0), (0, 1, 0), 60.0, 1.0)
+    vp = A.VolParams(0.5, 0.06, 0.5, 14.0)
+    w = h = 256
+    g, c = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM, engine=A.ENGINE_WAVEFRONT)
+    p, _ = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM)
+    err, err_p = rel_l1_per_channel(g, c), rel_l1_per_channel(g, p)
+    print("close-up beam: rel L1 vs oracle", err, "vs per-photon", err_p, "lit pixels", (g[..., 1] > 0).mean())
+    assert c[..., 1].sum() > 0 and (g[..., 1] > 0).mean() > 0.05
+    assert (err < 2e-5).all() and (err_p < 1e-4).all()
