@@ -9,7 +9,9 @@
 // steps over their 64 lanes, summing overlapping splats in LDS first.  The
 // RNG is drawn only by the trace (the beam estimator draws none per step),
 // and each step's splat is the same function as in the per-photon form, so
-// films differ only in the order of their fp32 sums.
+// films differ only in the order of their fp32 sums and in one product per
+// texel, reassociated (the tile sums weight * tent weight and applies the
+// segment's colour once per pixel).
 #include "spx_internal.hpp"
 #include "spx_paths.hpp"
 
