@@ -288,10 +288,12 @@ int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface
  * max(L_(k), 1e-9) where L_(k) is the k-th smallest, k = min(floor(fp32(w*h)
  * * q), w*h - 1).  The caller's exposure is c / *lum_q (1 in the viewer, 1.1
  * in prism_on_plane).  Exact selection on the device (radix select over the
- * float bits); device buffers in, host float out; synchronous. */
+ * float bits); device buffers in, host float out.  Its kernels and copies
+ * run on `stream`, ordered after earlier work there, and the call waits for
+ * that stream before it returns. */
 int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
                          uint32_t height, uint32_t total_photons, float target_n, float quantile,
-                         float *lum_q);
+                         float *lum_q, void *stream);
 
 /* ---- the floor-caustic product image (prism_on_plane.rs:16-185) ---------------- */
 /* box_blur (prism_on_plane.rs:40-74): separable box filter of radius r over

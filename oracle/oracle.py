@@ -66,6 +66,8 @@ _SIG = {
     "or_volume": (None, [_P(OrScene), _vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _u32, _u32, _vp,
                          _vp, _u32, _P(Stats)]),
     "or_phase_hg": (_f32, [_f32, _f32]),
+    "or_volume_beam_max_box": (C.c_uint64, [_P(OrScene), _vp, _vp, _u32, _u32, _u32, _u32, _vp, _vp, _u32, _vp,
+                                            _u32, _vp]),
     "or_photon_record_states": (None, [_P(OrScene), _vp, _vp, _u32, _u32, _f32, _u32, _vp, _vp]),
     "or_surface_rim": (None, [_P(OrScene), _vp, _u32, _u32, _vp, _vp]),
     "or_display_resolve": (None, [_vp, _vp, _u32, _u32, _u32, _f32, _f32, _f32, _vp]),
@@ -207,3 +209,15 @@ def surface_rim(scene, cam, w, h):
     depth = np.zeros((h, w), np.float32)
     lib().or_surface_rim(C.byref(sv), C.byref(cam.basis), w, h, rim.ctypes.data, depth.ctypes.data)
     return rim, depth
+
+
+def volume_beam_max_box(scene, cam, bm, w, h, n, photon_base, vp, zbuf, seed, opts, group=64):
+    """Largest bounding box (pixels) of `group` consecutive beam-estimator
+    steps of one segment, and the film the same photons give."""
+    sv = scene_view(scene)
+    img = np.zeros((h, w, 3), np.float32)
+    z = None if zbuf is None else np.ascontiguousarray(zbuf, np.float32)
+    a = lib().or_volume_beam_max_box(C.byref(sv), C.byref(cam.basis), C.byref(bm), w, h, n, photon_base,
+                                     C.byref(vp), None if z is None else z.ctypes.data, seed, C.byref(opts), group,
+                                     img.ctypes.data)
+    return int(a), img

@@ -931,7 +931,37 @@ typedef struct {
     float *img;
     float ill_scale;
     uint64_t segs, splats;
+    /* beam estimator, test hook: when box_group > 0, the largest pixel area
+     * of the bounding box of the footprints of box_group consecutive steps
+     * of one segment (steps k0 .. k0+box_group-1, those on the film and in
+     * front of the z-buffer), footprints clamped to the film as
+     * splat_transverse clamps them (volume.rs:178-218). */
+    uint32_t box_group;
+    uint64_t max_box;
 } vol_job;
+
+/* The footprint splat_transverse walks (volume.rs:180-190) joined into a
+ * group's box, and the box's area folded into the largest seen. */
+static void box_add(vol_job *j, int box[4], float s, float t, float radius_px) {
+    float cx = s * (float)j->w - 0.5f, cy = (1.0f - t) * (float)j->h - 0.5f;
+    float r = fmaxf(radius_px, 0.5f);
+    int x0 = (int)fmaxf(floorf(cx - r), 0.0f), y0 = (int)fmaxf(floorf(cy - r), 0.0f);
+    int x1 = (int)ceilf(cx + r), y1 = (int)ceilf(cy + r);
+    x1 = x1 < 0 ? 0 : (x1 > (int)j->w - 1 ? (int)j->w - 1 : x1);
+    y1 = y1 < 0 ? 0 : (y1 > (int)j->h - 1 ? (int)j->h - 1 : y1);
+    if (x0 < box[0]) box[0] = x0;
+    if (x1 > box[1]) box[1] = x1;
+    if (y0 < box[2]) box[2] = y0;
+    if (y1 > box[3]) box[3] = y1;
+}
+
+static void box_close(vol_job *j, int box[4]) {
+    if (box[1] >= 0 && box[1] >= box[0] && box[3] >= box[2]) {
+        uint64_t a = (uint64_t)(box[1] - box[0] + 1) * (uint64_t)(box[3] - box[2] + 1);
+        if (a > j->max_box) j->max_box = a;
+    }
+    box[0] = INT32_MAX; box[1] = -1; box[2] = INT32_MAX; box[3] = -1;
+}
 
 static inline uint32_t pix_index(float s, float t, uint32_t width, uint32_t height) {
     /* volume.rs:134-135: float -> usize truncation, then min(dim-1) */
@@ -1019,7 +1049,9 @@ static void *vol_worker(void *arg) {
                     m = (uint32_t)(mm < 1 ? 1 : (mm > 4096 ? 4096 : mm));
                 }
                 float ds = seg_len / (float)m;
+                int box[4] = {INT32_MAX, -1, INT32_MAX, -1};
                 for (uint32_t k = 0; k < m; ++k) {
+                    if (j->box_group && k > 0 && k % j->box_group == 0u) box_close(j, box);
                     float dist = ((float)k + 0.5f) * ds;
                     v3 p = add(ro, mul(rd, dist));
                     float s, t, dep;
@@ -1048,7 +1080,9 @@ static void *vol_worker(void *arg) {
                     }
                     splat_transverse(j->img, j->w, j->h, s, t, color, radius_px);
                     ++j->splats;
+                    if (j->box_group) box_add(j, box, s, t, radius_px);
                 }
+                if (j->box_group) box_close(j, box);
             }
             if (mi < 0) break;
             const or_material *mt = &sc->mats[mi];
@@ -1083,6 +1117,8 @@ void or_volume(const or_scene *s, const or_camera *cam, const or_beam *beam, uin
         j->i1 = photon_base + (uint32_t)((uint64_t)n * (t + 1) / threads);
         j->img = threads == 1 ? img : (float *)calloc(film, sizeof(float));
         j->segs = j->splats = 0;
+        j->box_group = 0;
+        j->max_box = 0;
     }
     if (threads == 1) vol_worker(&jobs[0]);
     else {
@@ -1100,6 +1136,26 @@ void or_volume(const or_scene *s, const or_camera *cam, const or_beam *beam, uin
         }
         stats->paths += n;
     }
+}
+
+/* Test hook: the beam estimator (volume.rs:233-329) over photons
+ * photon_base .. photon_base+n-1 on one thread, into img, returning the
+ * largest bounding-box area (pixels) of `group` consecutive steps of one
+ * segment -- the box a wave's LDS tile must hold when it takes a segment's
+ * steps `group` at a time. */
+uint64_t or_volume_beam_max_box(const or_scene *s, const or_camera *cam, const or_beam *beam, uint32_t w,
+                                uint32_t h, uint32_t n, uint32_t photon_base, const or_vol_params *vp,
+                                const float *zbuf, uint32_t seed, const or_opts *o, uint32_t group, float *img) {
+    vol_job j;
+    memset(&j, 0, sizeof j);
+    j.sc = s; j.cam = cam; j.beam = beam; j.w = w; j.h = h; j.seed = seed;
+    j.est = 1; j.vp = vp; j.zbuf = zbuf; j.o = o; j.ill_scale = or_illuminant_scale(o->illuminant);
+    j.i0 = photon_base;
+    j.i1 = photon_base + n;
+    j.img = img;
+    j.box_group = group ? group : 1u;
+    vol_worker(&j);
+    return j.max_box;
 }
 
 /* ------------------------------------------------------------------------- */

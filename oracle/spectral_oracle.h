@@ -191,6 +191,11 @@ void or_volume(const or_scene *s, const or_camera *cam, const or_beam *beam, uin
                const float *zbuf, uint32_t seed, uint32_t estimator, const or_opts *o,
                float *img, uint32_t threads, or_stats *stats);
 float or_phase_hg(float g, float cos_theta);
+/* Test hook: beam estimator on one thread into img; returns the largest
+ * bounding-box area of `group` consecutive steps of one segment. */
+uint64_t or_volume_beam_max_box(const or_scene *s, const or_camera *cam, const or_beam *beam, uint32_t w,
+                                uint32_t h, uint32_t n, uint32_t photon_base, const or_vol_params *vp,
+                                const float *zbuf, uint32_t seed, const or_opts *o, uint32_t group, float *img);
 
 /* Per-photon ray states (vol_photons.rs:932-991).  mode: 0 volume point /
  * legacy (4 dist draws per segment), 1 volume beam (no dist draws),

@@ -8,9 +8,10 @@ from .core import (Camera, Context, DeviceBuffer, Dielectric, Film, HipTracer, L
                    axis_box, beam, display_quantile, display_resolve, glass_n, lighttrace_trace, opts,
                    photon_records, resolve_srgb8,
                    render_volume, render_volumetric, render_volumetric_scene,
-                   render_volumetric_scene_point, rng_draws, screen, surface_rim, wedge)
+                   render_volumetric_scene_point, rng_draws, screen, surface_rim, triangular_prism, wedge)
 
 __all__ = ["abi", "Camera", "Context", "DeviceBuffer", "Dielectric", "Film", "HipTracer",
            "Lambertian", "Scene", "axis_box", "beam", "display_quantile", "display_resolve", "glass_n",
-           "lighttrace_trace", "opts", "photon_records", "resolve_srgb8", "render_volume", "render_volumetric", "render_volumetric_scene",
-           "render_volumetric_scene_point", "rng_draws", "screen", "surface_rim", "wedge"]
+           "lighttrace_trace", "opts", "photon_records", "resolve_srgb8", "render_volume", "render_volumetric",
+           "render_volumetric_scene", "render_volumetric_scene_point", "rng_draws", "screen", "surface_rim",
+           "triangular_prism", "wedge"]

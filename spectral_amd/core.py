@@ -124,18 +124,24 @@ class Context:
 
 
 class DeviceBuffer:
-    """A float32 device array (films, z-buffers)."""
+    """A float32 device array (films, z-buffers).  The initial fill runs on
+    `stream`; with the null stream it is waited for, so that work queued later
+    on a non-blocking stream (which does not wait for the null stream) sees
+    the filled buffer."""
 
-    def __init__(self, ctx: Context, n: int, fill: Optional[float] = 0.0):
+    def __init__(self, ctx: Context, n: int, fill: Optional[float] = 0.0, stream=None):
         self.ctx, self.n = ctx, int(n)
         self.ptr = ctx.malloc(max(4, 4 * self.n))
         if fill is not None:
-            ctx.fill(self.ptr, fill, self.n)
+            ctx.fill(self.ptr, fill, self.n, stream)
+            if stream is None:
+                ctx.sync(None)
 
     def clear(self, value: float = 0.0, stream=None) -> None:
         self.ctx.fill(self.ptr, value, self.n, stream)
 
     def download(self, stream=None) -> np.ndarray:
+        """Copies the buffer to the host after the work queued on `stream`."""
         out = np.empty(self.n, np.float32)
         self.ctx.download(self.ptr, out, stream)
         if stream is not None:
@@ -156,8 +162,8 @@ class DeviceBuffer:
 class Film(DeviceBuffer):
     """3*w*h interleaved XYZ sums, row 0 = top (AccumBuffer::sum, lib.rs:7-27)."""
 
-    def __init__(self, ctx: Context, width: int, height: int):
-        super().__init__(ctx, 3 * width * height)
+    def __init__(self, ctx: Context, width: int, height: int, stream=None):
+        super().__init__(ctx, 3 * width * height, stream=stream)
         self.width, self.height = width, height
 
     def image(self, stream=None) -> np.ndarray:
@@ -246,6 +252,19 @@ def axis_box(mn, mx) -> list:
             ((0, 0, 1), -mx[2]), ((0, 0, -1), mn[2])]
 
 
+def triangular_prism(size: float, depth: float) -> list:
+    """ConvexSolid::triangular_prism, geom.rs:89-103: three side planes with
+    outward normals at pi/2 + k*2pi/3 (angle, cos and sin in f32 as the
+    reference), d = -size (size is the inradius), and caps at z = +-depth/2."""
+    f = np.float32
+    sides = []
+    for k in range(3):
+        ang = f(np.pi / 2) + f(k) * f(2.0) * f(np.pi) / f(3.0)
+        sides.append(((float(np.cos(ang)), float(np.sin(ang)), 0.0), -float(size)))
+    half = float(f(depth) / f(2.0))
+    return sides + [((0, 0, 1), -half), ((0, 0, -1), -half)]
+
+
 def wedge(apex_deg: float, size: float, depth: float) -> list:
     """ConvexSolid::wedge, geom.rs:110-121 (cos/sin in f32 as the reference)."""
     a = np.float32(apex_deg) * np.float32(np.pi / 180.0)
@@ -331,7 +350,7 @@ def lighttrace_trace(ctx: Context, scene, scr: A.ScreenDesc, bm: A.BeamDesc, n: 
                      dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None) -> Film:
     """lighttrace::trace (lighttrace.rs:157-166): adds n photons into the screen film."""
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
-    film = film or Film(ctx, scr.width, scr.height)
+    film = film or Film(ctx, scr.width, scr.height, stream)
     o = opts(illuminant, dispersion)
     _check(ctx, A.lib().spx_lighttrace(ctx.handle, ds.handle, C.byref(scr), C.byref(bm), n,
                                        photon_base, seed, C.byref(o), engine, film.ptr, stream))
@@ -344,7 +363,7 @@ def render_volume(ctx: Context, scene, cam: Camera, bm: A.BeamDesc, width: int, 
                   film: Optional[Film] = None, photon_base: int = 0, illuminant: int = A.ILLUM_E,
                   dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None) -> Film:
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
-    film = film or Film(ctx, width, height)
+    film = film or Film(ctx, width, height, stream)
     vp = A.VolParams(sigma_s, sigma_t, g, max_dist)
     o = opts(illuminant, dispersion)
     _check(ctx, A.lib().spx_volume(ctx.handle, ds.handle, C.byref(cam.basis), C.byref(bm), width,
@@ -388,24 +407,24 @@ def photon_records(ctx: Context, scene, bm: A.BeamDesc, first: int, count: int, 
     return recs
 
 
-def surface_rim(ctx: Context, scene, cam: Camera, width: int, height: int):
+def surface_rim(ctx: Context, scene, cam: Camera, width: int, height: int, stream=None):
     """GpuPhotonFilm::recompute_surface (vol_photons.rs:685-714): rim XYZ + depth."""
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
-    rim = Film(ctx, width, height)
-    depth = DeviceBuffer(ctx, width * height)
+    rim = Film(ctx, width, height, stream)
+    depth = DeviceBuffer(ctx, width * height, stream=stream)
     _check(ctx, A.lib().spx_surface_rim(ctx.handle, ds.handle, C.byref(cam.basis), width, height,
-                                        rim.ptr, depth.ptr, None))
+                                        rim.ptr, depth.ptr, stream))
     return rim, depth
 
 
-def resolve_srgb8(ctx: Context, film: "Film", scale: float, exposure: float) -> np.ndarray:
+def resolve_srgb8(ctx: Context, film: "Film", scale: float, exposure: float, stream=None) -> np.ndarray:
     """xyz_to_linear_srgb + tonemap_to_u8 (cie.rs:121-144) of film * scale:
     (h, w, 3) uint8."""
-    out = DeviceBuffer(ctx, (3 * film.width * film.height + 3) // 4)
+    out = DeviceBuffer(ctx, (3 * film.width * film.height + 3) // 4, fill=None)
     try:
         _check(ctx, A.lib().spx_resolve_srgb8(ctx.handle, film.ptr, film.width, film.height, scale, exposure,
-                                              out.ptr, None))
-        raw = out.download()
+                                              out.ptr, stream))
+        raw = out.download(stream)
     finally:
         out.free()
     return raw.view(np.uint8)[:3 * film.width * film.height].reshape(film.height, film.width, 3).copy()
@@ -413,8 +432,8 @@ def resolve_srgb8(ctx: Context, film: "Film", scale: float, exposure: float) -> 
 
 def box_blur(ctx: Context, film: "Film", radius: int, out: Optional["Film"] = None, stream=None) -> "Film":
     """box_blur (prism_on_plane.rs:40-74) of a device film into `out` (new if None)."""
-    out = out or Film(ctx, film.width, film.height)
-    tmp = Film(ctx, film.width, film.height)
+    out = out or Film(ctx, film.width, film.height, stream)
+    tmp = Film(ctx, film.width, film.height, stream)
     try:
         _check(ctx, A.lib().spx_box_blur(ctx.handle, film.ptr, tmp.ptr, out.ptr, film.width, film.height, radius,
                                          stream))
@@ -428,8 +447,8 @@ def floor_surface(ctx: Context, scene, cam: Camera, floor: A.ScreenDesc, floor_s
                   ambient, background, tint, width: int, height: int, stream=None):
     """prism_on_plane's surface pass + z-buffer (prism_on_plane.rs:16-38, 125-154)."""
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
-    surface = Film(ctx, width, height)
-    depth = DeviceBuffer(ctx, width * height)
+    surface = Film(ctx, width, height, stream)
+    depth = DeviceBuffer(ctx, width * height, stream=stream)
     f3 = lambda v: (C.c_float * 3)(*v)
     _check(ctx, A.lib().spx_floor_surface(ctx.handle, ds.handle, C.byref(cam.basis), C.byref(floor), floor_sum.ptr,
                                           gain, f3(ambient), f3(background), f3(tint), width, height, surface.ptr,
@@ -437,40 +456,41 @@ def floor_surface(ctx: Context, scene, cam: Camera, floor: A.ScreenDesc, floor_s
     return surface, depth
 
 
-def composite_srgb8(ctx: Context, surface: "Film", vol: "Film", exposure: float) -> np.ndarray:
+def composite_srgb8(ctx: Context, surface: "Film", vol: "Film", exposure: float, stream=None) -> np.ndarray:
     """tonemap_to_u8(xyz_to_linear_srgb(surface + vol), exposure): (h, w, 3) uint8."""
-    out = DeviceBuffer(ctx, (3 * vol.width * vol.height + 3) // 4)
+    out = DeviceBuffer(ctx, (3 * vol.width * vol.height + 3) // 4, fill=None)
     try:
         _check(ctx, A.lib().spx_composite_srgb8(ctx.handle, surface.ptr, vol.ptr, vol.width, vol.height, exposure,
-                                                out.ptr, None))
-        raw = out.download()
+                                                out.ptr, stream))
+        raw = out.download(stream)
     finally:
         out.free()
     return raw.view(np.uint8)[:3 * vol.width * vol.height].reshape(vol.height, vol.width, 3).copy()
 
 
 def display_resolve(ctx: Context, vol: "Film", surface: Optional["Film"], total_photons: int, target_n: float,
-                    exposure: float, kradius: float = 2.0) -> np.ndarray:
+                    exposure: float, kradius: float = 2.0, stream=None) -> np.ndarray:
     """The viewer's display pass (spectral-viewer main.rs:131-174) of a device
     film: (h, w, 4) uint8 RGBA."""
-    out = DeviceBuffer(ctx, vol.width * vol.height)
+    out = DeviceBuffer(ctx, vol.width * vol.height, fill=None)
     try:
         _check(ctx, A.lib().spx_display_resolve(ctx.handle, vol.ptr, surface.ptr if surface is not None else None,
                                                 vol.width, vol.height, total_photons, target_n, exposure,
-                                                kradius, out.ptr, None))
-        raw = out.download()
+                                                kradius, out.ptr, stream))
+        raw = out.download(stream)
     finally:
         out.free()
     return raw.view(np.uint8).reshape(vol.height, vol.width, 4).copy()
 
 
 def display_quantile(ctx: Context, vol: "Film", surface: Optional["Film"], total_photons: int, target_n: float,
-                     quantile: float = 0.997) -> float:
-    """The exposure rules' luminance quantile (main.rs:614-644): exposure = c / this."""
+                     quantile: float = 0.997, stream=None) -> float:
+    """The exposure rules' luminance quantile (main.rs:614-644): exposure = c / this.
+    Runs after the work queued on `stream`."""
     q = C.c_float(0.0)
     _check(ctx, A.lib().spx_display_quantile(ctx.handle, vol.ptr, surface.ptr if surface is not None else None,
                                              vol.width, vol.height, total_photons, target_n, quantile,
-                                             C.byref(q)))
+                                             C.byref(q), stream))
     return q.value
 
 

@@ -874,27 +874,21 @@ int spx_display_resolve(spx_ctx *ctx, const float *vol_sum, const float *surface
                      "spx_display_resolve");
 }
 
-int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width, uint32_t height,
-                         uint32_t total_photons, float target_n, float quantile, float *lum_q) {
-    if (!ctx) return SPX_EINVAL;
-    int rc;
-    if ((rc = check_film(ctx, vol_sum, width, height, "spx_display_quantile"))) return rc;
-    if (!lum_q || !std::isfinite(target_n) || !(quantile >= 0.0f) || !(quantile <= 1.0f))
-        return set_error(ctx, SPX_EINVAL, "spx_display_quantile: bad output, target_n or quantile (0..1)");
-    const uint32_t n = width * height;
-    uint64_t rank = (uint64_t)((float)n * quantile);
-    if (rank > n - 1u) rank = n - 1u;
-    const float k = display_k(total_photons, target_n);
-    uint32_t *hist = nullptr;
-    hipError_t e = hipMalloc(&hist, 256 * sizeof(uint32_t));
-    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile alloc");
+// Radix select of the rank-th smallest composited luminance over `hist`
+// (256 words of device memory), stream-ordered on s: each pass clears the
+// histogram with a kernel, counts one key digit, copies the counts back and
+// waits for s.  Returns the selected float's key.
+static hipError_t quantile_select(const float *vol_sum, const float *surface, uint32_t n, float k, uint64_t rank,
+                                  uint32_t *hist, hipStream_t s, uint32_t *key_out) {
     uint32_t prefix = 0, mask = 0, h_hist[256];
-    for (int pass = 0; pass < 4 && e == hipSuccess; ++pass) {
+    hipError_t e = hipSuccess;
+    for (int pass = 0; pass < 4; ++pass) {
         const uint32_t shift = 24u - 8u * (uint32_t)pass;
-        e = hipMemset(hist, 0, 256 * sizeof(uint32_t));
-        if (e == hipSuccess) e = launch_lum_digit_hist(vol_sum, surface, n, k, prefix, mask, shift, hist, nullptr);
-        if (e == hipSuccess) e = hipMemcpy(h_hist, hist, sizeof(h_hist), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) break;
+        e = launch_fill(reinterpret_cast<float *>(hist), 0.0f, 256, s);  // 0.0f is all-zero bits
+        if (e == hipSuccess) e = launch_lum_digit_hist(vol_sum, surface, n, k, prefix, mask, shift, hist, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_hist, hist, sizeof(h_hist), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
         uint32_t b = 0;
         uint64_t below = 0;
         while (b < 255u && rank >= below + h_hist[b]) below += h_hist[b++];
@@ -902,12 +896,46 @@ int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surfac
         prefix |= b << shift;
         mask |= 0xFFu << shift;
     }
-    hipFree(hist);
-    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile");
-    const uint32_t u = (prefix & 0x80000000u) ? (prefix & 0x7FFFFFFFu) : ~prefix;
+    *key_out = prefix;
+    return hipSuccess;
+}
+
+static float key_to_lum(uint32_t key) {
+    const uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
     float L;
     memcpy(&L, &u, sizeof(L));
-    *lum_q = fmaxf(L, 1e-9f);
+    return fmaxf(L, 1e-9f);
+}
+
+static int quantile_args(spx_ctx *ctx, const float *vol_sum, uint32_t width, uint32_t height, float target_n,
+                         float quantile, const float *lum_q, const char *who) {
+    int rc;
+    if ((rc = check_film(ctx, vol_sum, width, height, who))) return rc;
+    if (!lum_q || !std::isfinite(target_n) || !(quantile >= 0.0f) || !(quantile <= 1.0f))
+        return set_error(ctx, SPX_EINVAL, std::string(who) + ": bad output, target_n or quantile (0..1)");
+    return SPX_OK;
+}
+
+static uint64_t quantile_rank(uint32_t n, float quantile) {
+    uint64_t rank = (uint64_t)((float)n * quantile);
+    return rank > n - 1u ? n - 1u : rank;
+}
+
+int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width, uint32_t height,
+                         uint32_t total_photons, float target_n, float quantile, float *lum_q, void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    int rc = quantile_args(ctx, vol_sum, width, height, target_n, quantile, lum_q, "spx_display_quantile");
+    if (rc) return rc;
+    const uint32_t n = width * height;
+    uint32_t *hist = nullptr;
+    hipError_t e = hipMalloc(&hist, 256 * sizeof(uint32_t));
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile alloc");
+    uint32_t key = 0;
+    e = quantile_select(vol_sum, surface, n, display_k(total_photons, target_n), quantile_rank(n, quantile), hist,
+                        (hipStream_t)stream, &key);
+    hipFree(hist);
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile");
+    *lum_q = key_to_lum(key);
     return SPX_OK;
 }
 

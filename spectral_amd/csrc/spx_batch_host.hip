@@ -114,7 +114,7 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
     // (one job, a few served requests) gets smaller items, down to 256
     // paths / photons, so that the persistent grid is filled and the round's
     // tail is short: four camera items per block (one 512^2 x 64 spp job:
-    // 17.2 against 16.4 G paths/s at two), two photon items (each ends in a
+    // 17.4 against 16.2 G paths/s at two), two photon items (each ends in a
     // flush of the block's deposit hash).
     if (!(cfg && (cfg->chunk_paths || cfg->chunk_photons))) {
         uint64_t paths = 0, photons = 0;
@@ -157,9 +157,10 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
                 job_cost[j] += (double)cost;
             }
         } else {
-            for (uint32_t p0 = 0; p0 < jb.n_photons; p0 += chunk_photons) {
-                uint32_t c = jb.n_photons - p0 < chunk_photons ? jb.n_photons - p0 : chunk_photons;
-                items.push_back({j, p0, c, 2u * c});
+            // 64-bit counter: n_photons near 2^32 must not wrap the loop
+            for (uint64_t p0 = 0; p0 < jb.n_photons; p0 += chunk_photons) {
+                uint32_t c = (uint32_t)std::min<uint64_t>(jb.n_photons - p0, chunk_photons);
+                items.push_back({j, (uint32_t)p0, c, 2u * c});
                 job_cost[j] += 2.0 * c;
             }
         }

@@ -39,7 +39,7 @@ def prism_on_plane(ctx: S.Context, width: int = 800, height: int = 600, n_causti
         floor = S.lighttrace_trace(ctx, sc, scr, bm, n_caustic, 0x0F1, engine=engine, stream=stream)
         films.append(floor)
         S.box_blur(ctx, floor, 2, out=floor, stream=stream)
-        floor_max = S.display_quantile(ctx, floor, None, 1, 1.0, 1.0)
+        floor_max = S.display_quantile(ctx, floor, None, 1, 1.0, 1.0, stream)
         floor_gain = float(np.float32(6.0) / np.float32(floor_max))
         # surface pass + z-buffer, prism_on_plane.rs:113-154
         cam = scenes.plane_camera(width, height)
@@ -54,11 +54,11 @@ def prism_on_plane(ctx: S.Context, width: int = 800, height: int = 600, n_causti
                                         stream=stream)
         films.append(vol)
         # exposure and tonemap, prism_on_plane.rs:166-182
-        p = S.display_quantile(ctx, vol, surface, 1, 1.0, 0.995)
+        p = S.display_quantile(ctx, vol, surface, 1, 1.0, 0.995, stream)
         exposure = float(np.float32(1.1) / np.float32(p))
-        rgb = S.composite_srgb8(ctx, surface, vol, exposure)
-        return PlaneImage(rgb, exposure, floor_gain, floor.image(), surface.image(),
-                          depth.download().reshape(height, width), vol.image())
+        rgb = S.composite_srgb8(ctx, surface, vol, exposure, stream)
+        return PlaneImage(rgb, exposure, floor_gain, floor.image(stream), surface.image(stream),
+                          depth.download(stream).reshape(height, width), vol.image(stream))
     finally:
         for f in films:
             f.free()

@@ -7,6 +7,7 @@ Reference fixtures:
   diff_gate_scene  spectral-gpu/tests/diff_gate.rs:11-17
   lambertian_scene spectral-gpu/tests/lambertian_gate.rs:20-26
   wedge_screen / wedge_beam  spectral-core/src/lighttrace.rs:188-207
+  scene_a_scene    spectral-core/tests/scene_a_prism.rs:66-77 (horizon sky)
 
 Benchmark workloads (sizes in each docstring):
   prism_scene / prism_beam / prism_camera  -- the DSOTM prism workload: equilateral
@@ -24,7 +25,7 @@ import math
 import numpy as np
 
 from . import _abi as A
-from .core import Camera, Dielectric, Lambertian, Scene, axis_box, beam, screen, wedge
+from .core import Camera, Dielectric, Lambertian, Scene, axis_box, beam, screen, triangular_prism, wedge
 
 F = np.float32
 
@@ -77,6 +78,20 @@ def lambertian_scene() -> Scene:
 def gate_camera():
     """The diff-gate camera (diff_gate.rs:102)."""
     return Camera.look_at((0, 0, 0), (0, 0, -1), (0, 1, 0), 50.0, 1.0)
+
+
+def scene_a_scene(dispersive: bool = True) -> Scene:
+    """scene_a_prism.rs:66-77: a triangular prism (inradius 1, depth 2), SF11
+    glass or a 0.8 Lambertian control, under a dim background (0.3) and a
+    bright sky above the horizon (4.0)."""
+    s = Scene(background=0.3, horizon=4.0)
+    s.add_solid(triangular_prism(1.0, 2.0), Dielectric(A.GLASS_SF11) if dispersive else Lambertian(0.8))
+    return s
+
+
+def scene_a_camera():
+    """scene_a_prism.rs:71-77."""
+    return Camera.look_at((0.0, 0.0, 4.0), (0.0, 0.0, 0.0), (0, 1, 0), 50.0, 1.0)
 
 
 def wedge_scene() -> Scene:

@@ -7,9 +7,7 @@ oracle at the configuration's film size, seed and sample count:
       Russian roulette from depth 3, seed 1 (full size)
   C2  prism on a Lambertian floor (albedo 0.8), 1920x1080 with the surface
       z-buffer, under D65, A and dispersion off, plus the floor caustic on a
-      1024x1024 screen: full film size, 2^22 of the configuration's 2^28
-      photons (the photons of a render are independent, so the first 2^22
-      are an exact prefix of the full run)
+      1024x1024 screen: full size, 2^28 photons (the oracle on 16 threads)
 
 Bar: relative L1 per XYZ channel below 0.5 %.  Each test prints its errors."""
 import ctypes as C
@@ -110,7 +108,7 @@ def test_config1_rainbow(ctx, oracle):
 
 @pytest.mark.parametrize("variant", ["D65", "A", "disp_off"])
 def test_config2_floor(ctx, oracle, variant):
-    w, h, n = 1920, 1080, 1 << 22
+    w, h, n = 1920, 1080, 1 << 28
     ill = A.ILLUM_A if variant == "A" else A.ILLUM_D65
     disp = variant != "disp_off"
     fs, fc, bm = scenes.prism_scene(floor=True), scenes.floor_camera(w, h), scenes.prism_beam()
@@ -139,4 +137,4 @@ def test_config2_floor(ctx, oracle, variant):
     g.free()
     zdev.free()
     assert c[..., 1].sum() > 0 and cc[..., 1].sum() > 0
-    _report(f"C2 floor 1920x1080 2^22 {variant}", errs)
+    _report(f"C2 floor 1920x1080 2^28 {variant}", errs)

@@ -106,9 +106,9 @@ def test_display_arguments_fail_loudly(ctx):
     rc = lib.spx_display_resolve(ctx.handle, film.ptr, None, 8, 8, 1, 1.0, 1.0, 40.0, out.ptr, None)
     assert rc == A.SPX_EINVAL and b"kradius" in lib.spx_last_error(ctx.handle)
     q = C.c_float(0.0)
-    rc = lib.spx_display_quantile(ctx.handle, film.ptr, None, 8, 8, 1, 1.0, 1.5, C.byref(q))
+    rc = lib.spx_display_quantile(ctx.handle, film.ptr, None, 8, 8, 1, 1.0, 1.5, C.byref(q), None)
     assert rc == A.SPX_EINVAL
-    rc = lib.spx_display_quantile(ctx.handle, None, None, 8, 8, 1, 1.0, 0.5, C.byref(q))
+    rc = lib.spx_display_quantile(ctx.handle, None, None, 8, 8, 1, 1.0, 0.5, C.byref(q), None)
     assert rc == A.SPX_EINVAL
     film.free()
     out.free()

@@ -19,6 +19,7 @@ pytestmark = pytest.mark.gpu
 
 FILM_TOL = 5e-3   # relative L1 per XYZ channel
 PATH_RTOL = 1e-6  # per-photon states, relative
+BEAM_TILE_PIX = 3072  # the wavefront beam kernel's LDS tile (kBeamTilePix, k_beam.hip)
 
 
 def test_rng_parity(ctx, oracle):
@@ -220,6 +221,34 @@ def test_camera_film(ctx, oracle, scene_name, engine):
     assert st["frac_above_10x_mean"] < 0.005 and st["p99"] < 0.05 and st["energy_rel"] < 0.01
 
 
+@pytest.mark.parametrize("engine", [A.ENGINE_PLAIN, A.ENGINE_WAVEFRONT])
+def test_scene_a_horizon_camera_film(ctx, oracle, engine):
+    # scene_a_prism.rs:66-90 on the device: the horizon sky (scene.rs:44-50)
+    # lights escaped rays looking up; SF11 prism and its diffuse control,
+    # 128x128, 64 spp, seed 11 -- films against the oracle, and the
+    # reference's gate (glass chroma > 0.05 and > 2x the control) on them
+    cam, spread = scenes.scene_a_camera(), []
+    for disp in (True, False):
+        sc = scenes.scene_a_scene(disp)
+        tr = S.HipTracer(ctx, sc, cam, 128, 128, A.ILLUM_D65, 11, engine=engine)
+        tr.accumulate(64)
+        g = tr.sum()
+        c, _ = oracle.camera_accumulate(sc, cam, 128, 128, 64, 0, 11, opts(A.ILLUM_D65), threads=8)
+        err = rel_l1_per_channel(g, c)
+        print("scene A", "glass" if disp else "diffuse", engine, "rel L1", err,
+              "bit-identical", np.array_equal(g, c))
+        assert (err < FILM_TOL).all()
+        up = g[:16, :16, 1].mean() / 64  # upper-left corner: sky above the horizon
+        down = g[-16:, :16, 1].mean() / 64
+        assert up > 10 * down > 0
+        px = (g / np.float32(64)).reshape(-1, 3)
+        s = px.sum(1)
+        ok = s > 1e-5
+        spread.append(float(np.sqrt((px[ok, 0] / s[ok] - 0.3127) ** 2 + (px[ok, 1] / s[ok] - 0.3290) ** 2).max()))
+        tr.film.free()
+    assert spread[0] > 0.05 and spread[0] > 2.0 * spread[1]
+
+
 def test_camera_wavefront_equals_plain_bitwise(ctx):
     # The wavefront resolve sums each pixel's samples in sample order, as the
     # plain kernel does, so the two films are identical.
@@ -371,6 +400,10 @@ def test_beam_wavefront_groups_larger_than_the_tile(ctx, oracle):
     cam = S.Camera.look_at((-1.93, -0.52, 0.75), (-1.93, -0.52, 0.0), (0, 1, 0), 60.0, 1.0)
     vp = A.VolParams(0.5, 0.06, 0.5, 14.0)
     w = h = 256
+    # the oracle's own steps: some group of 64 has a box above the tile
+    box, _ = oracle.volume_beam_max_box(sc, cam, bm, w, h, 200, 0, vp, None, 5, opts(A.ILLUM_E), 64)
+    print("largest 64-step box", box, "pixels")
+    assert box > BEAM_TILE_PIX
     g, c = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM, engine=A.ENGINE_WAVEFRONT)
     p, _ = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM)
     err, err_p = rel_l1_per_channel(g, c), rel_l1_per_channel(g, p)

@@ -95,3 +95,23 @@ def test_prism_on_plane_image_matches_the_oracle(ctx, oracle):
     print(f"image: max byte diff {d.max()}, pixels differing by more than one {(d > 1).any(-1).mean():.2e}")
     assert (d > 1).any(-1).mean() < 5e-3
     assert img.rgb.max() > 0 and vol[..., 1].sum() > 0
+
+
+def test_prism_on_plane_on_a_stream_equals_the_null_stream_run(ctx):
+    # Every pass of the product image runs on the caller's stream: a
+    # non-blocking stream gives the null-stream image and exposure (the
+    # quantile, composite and downloads wait for that stream's renders).
+    w, h, side, n_c, n_v = 200, 150, 128, 1 << 18, 1 << 16
+    ref = product.prism_on_plane(ctx, w, h, n_c, n_v, side)
+    st = ctx.stream()
+    try:
+        got = product.prism_on_plane(ctx, w, h, n_c, n_v, side, stream=st)
+    finally:
+        ctx._lib.spx_stream_destroy(ctx.handle, st)
+    ev = rel_l1_per_channel(got.volume, ref.volume)
+    print(f"stream vs null stream: exposure {got.exposure} / {ref.exposure}, volume rel L1 {ev}")
+    assert abs(got.exposure / ref.exposure - 1) < 1e-5 and abs(got.floor_gain / ref.floor_gain - 1) < 1e-5
+    assert (ev < 1e-5).all()
+    assert np.array_equal(got.depth, ref.depth) and np.array_equal(got.surface, ref.surface)
+    d = np.abs(got.rgb.astype(np.int32) - ref.rgb.astype(np.int32))
+    assert d.max() <= 1
