@@ -112,6 +112,10 @@ def test_prism_on_plane_on_a_stream_equals_the_null_stream_run(ctx):
     print(f"stream vs null stream: exposure {got.exposure} / {ref.exposure}, volume rel L1 {ev}")
     assert abs(got.exposure / ref.exposure - 1) < 1e-5 and abs(got.floor_gain / ref.floor_gain - 1) < 1e-5
     assert (ev < 1e-5).all()
-    assert np.array_equal(got.depth, ref.depth) and np.array_equal(got.surface, ref.surface)
-    d = np.abs(got.rgb.astype(np.int32) - ref.rgb.astype(np.int32))
-    assert d.max() <= 1
+    # the depth pass is deterministic; the surface shades the floor film,
+    # whose fp32 atomic sums vary in their last bits from run to run
+    assert bool(np.array_equal(got.depth, ref.depth)), "depth differs"
+    sd = float(np.abs(got.surface - ref.surface).max())
+    assert sd < 1e-5 * max(float(np.abs(ref.surface).max()), 1.0), f"surface differs by {sd}"
+    d = int(np.abs(got.rgb.astype(np.int32) - ref.rgb.astype(np.int32)).max())
+    assert d <= 1, f"image differs by {d}"
