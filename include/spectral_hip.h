@@ -295,6 +295,21 @@ int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surfac
                          uint32_t height, uint32_t total_photons, float target_n, float quantile,
                          float *lum_q, void *stream);
 
+/* No counterpart in the reference: spx_display_quantile for callers that
+ * cannot allocate device memory (tight per-frame display loops, harnesses
+ * that forbid hipMalloc).  Same arguments and result; it calls no hipMalloc
+ * or hipFree: the histograms and the select's state live in a fixed 4 KiB
+ * device array of the library (one per GPU, held until the process ends),
+ * cleared by the call itself, and the four radix passes decide their digits
+ * on the device, so the only host round trip is the final 4-byte result.
+ * Films up to 2^28 pixels (SPX_ECAP above: the counts are 32-bit).  Every
+ * call on one GPU shares that GPU's array: calls that overlap on one GPU
+ * (two streams, two threads) race and must be serialised by the caller.
+ * Waits for `stream` before it returns. */
+int spx_display_quantile_noalloc(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
+                                 uint32_t height, uint32_t total_photons, float target_n, float quantile,
+                                 float *lum_q, void *stream);
+
 /* ---- the floor-caustic product image (prism_on_plane.rs:16-185) ---------------- */
 /* box_blur (prism_on_plane.rs:40-74): separable box filter of radius r over
  * a 3*w*h XYZ buffer, clamp-to-edge; the horizontal pass into tmp, the

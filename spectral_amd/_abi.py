@@ -154,6 +154,7 @@ SIGNATURES = {
     "spx_composite_srgb8": (_i32, [_vp, _vp, _vp, _u32, _u32, _f32, _vp, _vp]),
     "spx_display_resolve": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _f32, _f32, _f32, _vp, _vp]),
     "spx_display_quantile": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _f32, _f32, _P(_f32), _vp]),
+    "spx_display_quantile_noalloc": (_i32, [_vp, _vp, _vp, _u32, _u32, _u32, _f32, _f32, _P(_f32), _vp]),
     "spx_rng_draws": (_i32, [_vp, _u32, _u32, _u32, _P(_u32)]),
     "spx_glass_n": (_i32, [_vp, _u32, _P(_f32), _u32, _P(_f32)]),
     "spx_batch_run": (_i32, [_vp, _P(Job), _u32, _P(BatchCfg), _vp]),

@@ -287,6 +287,90 @@ hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n
     return hipGetLastError();
 }
 
+// The same radix select with its decisions kept on the device, in a
+// file-scope scratch array (no allocation per call): four 256-bin
+// histograms, then {prefix, mask, rank low word, rank high word}.  Every
+// call on a device shares that device's copy; calls that overlap on one
+// device race.  Host code reaches the array only through the address
+// hipGetSymbolAddress returns for the current device.
+constexpr uint32_t kQuantileHist = 4 * 256;
+constexpr uint32_t kQuantileWords = kQuantileHist + 4;
+static __device__ __align__(256) uint32_t scratch[kQuantileWords];
+
+int quantile_scratch_address(uint32_t **out) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return (int)e;
+    *out = static_cast<uint32_t *>(p);
+    return 0;
+}
+
+__global__ __launch_bounds__(256) void k_quantile_init(uint32_t *q, unsigned long long rank) {
+    for (uint32_t i = threadIdx.x; i < kQuantileHist; i += blockDim.x) q[i] = 0u;
+    if (threadIdx.x == 0) {
+        q[kQuantileHist + 0] = 0u;  // prefix
+        q[kQuantileHist + 1] = 0u;  // mask
+        q[kQuantileHist + 2] = (uint32_t)rank;
+        q[kQuantileHist + 3] = (uint32_t)(rank >> 32);
+    }
+}
+
+// One pass: the histogram of the digit at `shift` among keys matching the
+// prefix selected so far.
+__global__ __launch_bounds__(256) void k_lum_digit_hist_dev(const float *vol, const float *surf, uint32_t n,
+                                                            float k, uint32_t shift, uint32_t *q, uint32_t pass) {
+    __shared__ uint32_t sh[256];
+    sh[threadIdx.x] = 0;
+    const uint32_t prefix = __builtin_amdgcn_readfirstlane(q[kQuantileHist + 0]);
+    const uint32_t mask = __builtin_amdgcn_readfirstlane(q[kQuantileHist + 1]);
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float L = vol[3 * (size_t)i + 1] * k;
+        if (surf) L = L + surf[3 * (size_t)i + 1];
+        const uint32_t key = float_order_key(L);
+        if ((key & mask) == prefix) atomicAdd(&sh[(key >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (sh[threadIdx.x]) atomicAdd(&q[pass * 256 + threadIdx.x], sh[threadIdx.x]);
+}
+
+// The digit holding the rank-th key of the pass's histogram: one lane walks
+// the 256 bins (as the host loop of the allocating form does).
+__global__ void k_quantile_select(uint32_t *q, uint32_t pass, uint32_t shift) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const uint32_t *h = q + pass * 256;
+    unsigned long long rank = (unsigned long long)q[kQuantileHist + 2] |
+                              ((unsigned long long)q[kQuantileHist + 3] << 32);
+    uint32_t b = 0;
+    unsigned long long below = 0;
+    while (b < 255u && rank >= below + h[b]) below += h[b++];
+    rank -= below;
+    q[kQuantileHist + 0] |= b << shift;
+    q[kQuantileHist + 1] |= 0xFFu << shift;
+    q[kQuantileHist + 2] = (uint32_t)rank;
+    q[kQuantileHist + 3] = (uint32_t)(rank >> 32);
+}
+
+// The four passes of the select, stream-ordered, with no host round trip;
+// the selected key ends in q[kQuantileHist] (the prefix word).
+hipError_t launch_quantile_select_dev(const float *vol, const float *surf, uint32_t n, float k,
+                                      unsigned long long rank, uint32_t *q, hipStream_t s) {
+    hipLaunchKernelGGL(k_quantile_init, dim3(1), dim3(256), 0, s, q, rank);
+    hipError_t e = hipGetLastError();
+    uint32_t blocks = (n + 255) / 256;
+    blocks = blocks < 1 ? 1 : (blocks > 1024 ? 1024 : blocks);
+    for (uint32_t pass = 0; pass < 4 && e == hipSuccess; ++pass) {
+        const uint32_t shift = 24u - 8u * pass;
+        hipLaunchKernelGGL(k_lum_digit_hist_dev, dim3(blocks), dim3(256), 0, s, vol, surf, n, k, shift, q, pass);
+        if ((e = hipGetLastError()) != hipSuccess) break;
+        hipLaunchKernelGGL(k_quantile_select, dim3(1), dim3(64), 0, s, q, pass, shift);
+        e = hipGetLastError();
+    }
+    return e;
+}
+
+uint32_t quantile_scratch_key_offset() { return kQuantileHist; }
+
 // box_blur, prism_on_plane.rs:40-74: one pass of the separable box filter
 // (dx = 1: along rows, else along columns), clamp-to-edge, taps summed in
 // order then scaled by 1/(2r+1).

@@ -410,6 +410,12 @@ int spx_ctx_create(int device, spx_ctx **out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    // this device's address of the exposure quantile's scratch array (the
+    // entry point that allocates nothing uses it)
+    if (quantile_scratch_address(&ctx->quantile_scratch) != 0 || !ctx->quantile_scratch) {
+        delete ctx;
+        return SPX_EDEVICE;
+    }
     float cmf[kTblN][3], d65[kTblN], a[kTblN];
     if (parse_table(kCmfCsv, 3, &cmf[0][0]) || parse_table(kD65Csv, 1, d65) || parse_table(kACsv, 1, a)) {
         delete ctx;
@@ -935,6 +941,28 @@ int spx_display_quantile(spx_ctx *ctx, const float *vol_sum, const float *surfac
                         (hipStream_t)stream, &key);
     hipFree(hist);
     if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile");
+    *lum_q = key_to_lum(key);
+    return SPX_OK;
+}
+
+int spx_display_quantile_noalloc(spx_ctx *ctx, const float *vol_sum, const float *surface, uint32_t width,
+                                 uint32_t height, uint32_t total_photons, float target_n, float quantile, float *lum_q,
+                                 void *stream) {
+    if (!ctx) return SPX_EINVAL;
+    // films above 2^28 pixels are refused here (SPX_ECAP): the scratch
+    // histograms count in 32 bits
+    int rc = quantile_args(ctx, vol_sum, width, height, target_n, quantile, lum_q, "spx_display_quantile_noalloc");
+    if (rc) return rc;
+    const uint32_t n = width * height;
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t *q = ctx->quantile_scratch;
+    hipError_t e = launch_quantile_select_dev(vol_sum, surface, n, display_k(total_photons, target_n),
+                                              quantile_rank(n, quantile), q, s);
+    uint32_t key = 0;
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(&key, q + quantile_scratch_key_offset(), sizeof(key), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return check_hip(ctx, e, "spx_display_quantile_noalloc");
     *lum_q = key_to_lum(key);
     return SPX_OK;
 }

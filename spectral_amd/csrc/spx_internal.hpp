@@ -53,6 +53,7 @@ struct spx_ctx {
     int num_cus = 0;
     float4 *tbl_dev = nullptr;  // [2][71]: D65 rows then A rows (w = S)
     float tbl_host[2][spx::kTblN][4];
+    uint32_t *quantile_scratch = nullptr;  // the device's copy of the quantile's file-scope scratch array
     double ill_mean[2];
     std::string last_error;
     void *scratch = nullptr;   // grow-only work buffer for the wavefront queues
@@ -126,6 +127,14 @@ hipError_t launch_composite_srgb8(const float *surf, const float *vol, uint32_t 
                                   uint8_t *rgb, hipStream_t s);
 hipError_t launch_display_resolve(const float *vol, const float *surf, uint32_t w, uint32_t h, float k,
                                   int radius, float inv2s2, float exposure, uint8_t *rgba, hipStream_t s);
+// The exposure quantile over the file-scope scratch array of k_plain.hip:
+// its device address for the current device (0 or a hipError_t), the four
+// radix passes with their decisions kept on the device, and the word of the
+// array that ends holding the selected key.
+int quantile_scratch_address(uint32_t **out);
+hipError_t launch_quantile_select_dev(const float *vol, const float *surf, uint32_t n, float k,
+                                      unsigned long long rank, uint32_t *q, hipStream_t s);
+uint32_t quantile_scratch_key_offset();
 hipError_t launch_lum_digit_hist(const float *vol, const float *surf, uint32_t n, float k, uint32_t prefix,
                                  uint32_t mask, uint32_t shift, uint32_t *hist, hipStream_t s);
 hipError_t launch_rng_draws(uint32_t key, uint32_t seed, uint32_t n, uint32_t *out, hipStream_t s);
