@@ -40,6 +40,10 @@ from spectral_amd.jobs import TUNED_MIXED512  # noqa: E402  spx_autotune's pick 
 
 UNIT = "Mphotons/s + Mpaths/s"
 HBM_PEAK_GBS = 8000.0
+# VALU lane operations per second: 256 CUs x 4 SIMDs x 16 lanes per clock
+# (a wave64 instruction issues over 4 clocks; packed fp32 is compiled out)
+# x 2.4 GHz peak engine clock.
+VALU_LANE_PEAK = 256 * 4 * 16 * 2.4e9
 # Algorithmic bytes (per unit of work): a camera path segment moves a 44 B
 # state in and out (88 B) and a finished path deposits one XYZ (12 B); a
 # photon segment moves a 40 B state in and out (80 B) and each film deposit
@@ -87,60 +91,116 @@ def profile_figure(summary: str, key: str):
     return None
 
 
+SQ_PROFILE = "batched_step_mixed512_sq_counters.md"
+HBM_PROFILE = "batched_step_mixed512_hbm_counters.md"
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload (FETCH_SIZE and
     WRITE_SIZE, copy-calibrated)."""
-    return profile_figure("batched_step_mixed512_hbm_counters.md", f"traffic_bytes_per_launch[{kernel}]")
+    return profile_figure(HBM_PROFILE, f"traffic_bytes_per_launch[{kernel}]")
 
 
-def cpu_baseline(paths_total, photons_total, target_s=12.0):
-    """The oracle on this host: camera paths on all cores (the reference's
-    rayon loop), volume photons single-threaded (volume.rs:118), on bounded
-    samples of the two job classes.  Also returns the oracle's segment and
-    deposit counts on those samples (the work model of the roofline)."""
+def _median_time(fn, reps=5):
+    """One warm-up run, then the median wall time of `reps` timed runs (s)."""
+    fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline(paths_total, photons_total, reps=5):
+    """The oracle (the C restatement of spectral-core) on this host, each
+    figure one warm-up run plus the median of `reps`: the configurations
+    one by one -- DSOTM prism 2^20 photons at 320x180 on one thread (the
+    reference's single-threaded photon loop, volume.rs:118) and on all
+    threads; rainbow 512^2 x 64 spp on all threads (its rayon loop,
+    tracer.rs:116-134); the prism-on-floor volume at 1920x1080 per variant
+    on a 2^18-photon sample on one thread and a 2^22-photon sample on all
+    threads (photons are independent, so a rate measured on the first n of
+    a render's photons is its rate) -- and the mixed stream at the rainbow's
+    all-thread path rate and the single-threaded rate of its volume class
+    (three prism rotations at 512x512).  Also returns the oracle's segment
+    and deposit counts on those runs (the roofline's work model)."""
     from oracle import oracle as O
     from spectral_amd import _abi as A
     from spectral_amd import jobs, scenes
+    from spectral_amd.core import opts
     cores = min(os.cpu_count() or 1, 16)
-    spec = jobs.JobSpec(A.JOB_CAMERA, 256, spp=2, seed=1)
-    sc, cam, _ = jobs.job_geometry(spec)
-    t0 = time.perf_counter()
-    O.camera_accumulate(sc, cam, 256, 256, 2, 0, 1, jobs.job_opts(spec), threads=cores)
-    t_cam = time.perf_counter() - t0
-    spp = max(1, int(2 * (target_s / 2) / max(t_cam, 1e-3)))
-    spp = min(spp, 64)
-    t0 = time.perf_counter()
-    _, cst = O.camera_accumulate(sc, cam, 256, 256, spp, 2, 1, jobs.job_opts(spec), threads=cores)
-    t_cam = time.perf_counter() - t0
-    cam_rate = 256 * 256 * spp / t_cam
     vp = A.VolParams(**scenes.HAZE)
+    rows, counts = {}, {}
 
-    def volume_samples(n):
-        segs = splats = paths = 0
-        t0 = time.perf_counter()
-        for rot, off in VOLUME_SAMPLES:
-            sp = jobs.JobSpec(A.JOB_VOLUME, 512, n_photons=n, rotation=rot, offset=off, seed=2)
-            sc, cam, bm = jobs.job_geometry(sp)
-            _, st = O.volume(sc, cam, bm, 512, 512, n, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp),
-                             threads=1)
-            segs, splats, paths = segs + st.segments, splats + st.splats, paths + st.paths
-        return time.perf_counter() - t0, segs, splats, paths
+    def volume(sc, cam, bm, w, h, n, ill, disp, threads, zbuf=None, key=None):
+        def run():
+            _, st = O.volume(sc, cam, bm, w, h, n, 0, vp, zbuf, 0, A.EST_POINT, opts(ill, disp), threads=threads)
+            if key:
+                counts[key] = {"seg": st.segments / st.paths, "splat": st.splats / st.paths}
+        return n / _median_time(run, reps) / 1e6
 
-    t1, *_ = volume_samples(1 << 12)
-    n = min(1 << 21, max(1 << 12, int((1 << 12) * (target_s / 2) / max(t1, 1e-3))))
-    t_ph, segs, splats, nph = volume_samples(n)
-    ph_rate = nph / t_ph
+    # DSOTM prism, the whole configuration
+    sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera(320, 180)
+    rows["prism_2^20_320x180"] = {
+        "mphotons_s_1thread": round(volume(sc, cam, bm, 320, 180, 1 << 20, A.ILLUM_D65, True, 1, key="prism"), 4),
+        "mphotons_s_all_threads": round(volume(sc, cam, bm, 320, 180, 1 << 20, A.ILLUM_D65, True, cores), 4)}
+    # rainbow, the whole configuration
+    rs, rc = scenes.rainbow_scene(), scenes.rainbow_camera(512, 512)
+
+    def rainbow():
+        _, st = O.camera_accumulate(rs, rc, 512, 512, 64, 0, 1, opts(A.ILLUM_D65, True, 8, 3), threads=cores)
+        counts["rainbow"] = {"seg": st.segments / st.paths}
+    cam_rate = 512 * 512 * 64 / _median_time(rainbow, reps)
+    rows["rainbow_512x512x64"] = {"mpaths_s_all_threads": round(cam_rate / 1e6, 4)}
+    # prism on floor, per variant, on the first 2^18 / 2^22 photons
+    fs, fc = scenes.prism_scene(floor=True), scenes.floor_camera(1920, 1080)
+    _, depth = O.surface_rim(fs, fc, 1920, 1080)
+    z = depth.ravel()
+    for label, ill, disp in (("D65", A.ILLUM_D65, True), ("A", A.ILLUM_A, True), ("disp_off", A.ILLUM_D65, False)):
+        rows[f"floor_volume_1920x1080_{label}"] = {
+            "mphotons_s_1thread": round(volume(fs, fc, bm, 1920, 1080, 1 << 18, ill, disp, 1, z,
+                                               key="floor_volume" if label == "D65" else None), 4),
+            "mphotons_s_all_threads": round(volume(fs, fc, bm, 1920, 1080, 1 << 22, ill, disp, cores, z), 4),
+            "sample_photons": {"1thread": 1 << 18, "all_threads": 1 << 22}}
+    # the floor caustic's work model (lighttrace onto the floor screen)
+    _, st = O.lighttrace(scenes.prism_scene(), scenes.floor_screen(1024, 1024), bm, 1 << 16, 0, 0x0F1,
+                         opts(A.ILLUM_D65), threads=cores)
+    counts["floor_caustic"] = {"seg": st.segments / st.paths, "splat": st.splats / st.paths}
+    # the mixed stream's volume class, one thread
+    tot = {"seg": 0, "splat": 0, "n": 0}
+
+    def mixed_volume(n):
+        def run():
+            tot.update(seg=0, splat=0, n=0)
+            for rot, off in VOLUME_SAMPLES:
+                sp = jobs.JobSpec(A.JOB_VOLUME, 512, n_photons=n, rotation=rot, offset=off, seed=2)
+                s, c, b = jobs.job_geometry(sp)
+                _, st = O.volume(s, c, b, 512, 512, n, 0, vp, None, 2, A.EST_POINT, jobs.job_opts(sp), threads=1)
+                tot["seg"] += st.segments
+                tot["splat"] += st.splats
+                tot["n"] += st.paths
+        return run
+    n_mix = 1 << 17
+    ph_rate = 3 * n_mix / _median_time(mixed_volume(n_mix), reps)
+    # the mixed stream's camera class is rainbow renders: its paths per
+    # segment from a sample of the stream's own job shape
+    spec = jobs.JobSpec(A.JOB_CAMERA, 256, spp=2, seed=1)
+    s, c, _ = jobs.job_geometry(spec)
+    _, cst = O.camera_accumulate(s, c, 256, 256, 2, 2, 1, jobs.job_opts(spec), threads=cores)
     step_s = paths_total / cam_rate + photons_total / ph_rate
-    work = {"seg_per_path": cst.segments / cst.paths, "seg_per_photon": segs / nph,
-            "splat_per_photon": splats / nph, "configs": config_work(O)}
+    work = {"seg_per_path": cst.segments / cst.paths, "seg_per_photon": tot["seg"] / tot["n"],
+            "splat_per_photon": tot["splat"] / tot["n"],
+            "configs": {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in counts.items()}}
     return {"value": round((paths_total + photons_total) / step_s / 1e6, 3), "unit": UNIT,
             "cores": cores, "kind": "port",
-            "sample": (f"oracle (C restatement of spectral-core): rainbow 256x256x{spp} spp on {cores} "
-                       f"threads = {cam_rate / 1e6:.3f} Mpaths/s; prism-on-floor volume 3 x {n} photons "
-                       f"(3 prism rotations) at 512x512 on 1 thread = {ph_rate / 1e6:.3f} Mphotons/s; "
-                       f"value = the mixed step's paths and photons at those rates"),
+            "sample": (f"oracle (C restatement of spectral-core), warm-up + median of {reps}: the mixed "
+                       f"stream's paths at the rainbow 512x512x64 rate on {cores} threads "
+                       f"({cam_rate / 1e6:.3f} Mpaths/s) and its photons at the single-threaded rate of "
+                       f"3 x {n_mix} prism-on-floor photons at 512x512 ({ph_rate / 1e6:.3f} Mphotons/s); "
+                       f"per-configuration rates under `configs`"),
             "mpaths_s": round(cam_rate / 1e6, 4), "mphotons_s_1thread": round(ph_rate / 1e6, 4),
-            "host": host_cpu()}, work
+            "configs": rows, "host": host_cpu()}, work
 
 
 def host_cpu():
@@ -159,8 +219,8 @@ def host_cpu():
 
 def config_work(O):
     """Segments and deposits per unit of each single-job configuration, from
-    the oracle's counters on small samples of the same scenes (the film
-    sizes shrunk where they only set the pixel grid)."""
+    the oracle's counters on small samples of the same scenes (used when the
+    CPU leg is skipped)."""
     from spectral_amd import _abi as A
     from spectral_amd import scenes
     from spectral_amd.core import opts
@@ -184,9 +244,13 @@ def config_work(O):
 
 
 def config_roofline(out, work, a_peak):
-    """Per-configuration roofline: algorithmic HBM bytes (the
-    wavefront state model) and deposit atomics over each measured rate,
-    against 8 TB/s and the measured atomic peak."""
+    """Per-configuration model fractions: the wavefront state model's HBM
+    bytes over 8 TB/s for every form, and for the plain photon kernels --
+    whose every deposit is three global fp32 atomics, so the oracle's
+    deposit count is their atomic count -- the atomics over the measured
+    scattered-atomic peak.  The LDS-aggregated forms (wavefront, batched)
+    issue far fewer global atomics than deposits, and camera kernels none
+    per path, so no atomic fraction is given for them."""
     rows = {}
     for key, rate in out.items():
         if not key.endswith(("_mphotons_s", "_mpaths_s")):
@@ -198,14 +262,15 @@ def config_roofline(out, work, a_peak):
             continue
         wm = work[cls]
         units = rate * 1e6  # per second
+        row = {}
         if cls == "rainbow":
             bytes_s = units * (wm["seg"] * CAM_SEG_BYTES + CAM_PATH_BYTES)
-            atom_s = units * 3
         else:
             bytes_s = units * (wm["seg"] * PH_SEG_BYTES + wm["splat"] * SPLAT_BYTES)
-            atom_s = units * wm["splat"] * 3
-        rows[key[:key.rfind("_m")]] = {"hbm_frac": round(bytes_s / 1e9 / HBM_PEAK_GBS, 4),
-                                       "atomic_frac": round(atom_s / 1e9 / a_peak, 4) if a_peak else None}
+            if "_plain_" in key and a_peak:
+                row["atomic_frac"] = round(units * wm["splat"] * 3 / 1e9 / a_peak, 4)
+        row["hbm_model_frac"] = round(bytes_s / 1e9 / HBM_PEAK_GBS, 4)
+        rows[key[:key.rfind("_m")]] = row
     return rows
 
 
@@ -319,27 +384,47 @@ def photon_kernel_name(t) -> str:
     return f"k_batch_photons<{mode}, {1024 if wide else 256}>"
 
 
-def kernel_roofline(name, alg_bytes_step, ms_step, launches_step):
-    """Algorithmic HBM bytes of a kernel group over its measured time."""
+def kernel_roofline(name, alg_bytes_step, ms_step, launches_step, bound="valu"):
+    """The roofline of one kernel group of the step.
+
+    bound "valu" (the persistent trace kernels, which keep path state in
+    registers and issue VALU work on most SIMD cycles): achieved = the VALU
+    lane operations of one launch (SQ_INSTS_VALU x lanes active per
+    instruction, counted by rocprofv3 on this workload) over the launch's
+    duration measured here, against the VALU lane peak; the algorithmic HBM
+    bytes over the duration are kept as `hbm_model`.  bound "hbm" (the bin
+    passes, which stream records): algorithmic deposit bytes over the
+    duration against 8 TB/s."""
     per_launch = alg_bytes_step / max(launches_step, 1)
     ms_launch = ms_step / max(launches_step, 1)
-    achieved = per_launch / (ms_launch / 1e3) / 1e9 if ms_launch > 0 else 0.0
-    out = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(name), "kernel": name,
-           "kernel_ms_per_launch": round(ms_launch, 3), "launches_per_step": launches_step,
-           "alg_bytes_per_launch": round(per_launch)}
-    # What actually bounds it: the share of SIMD cycles issuing VALU work
-    # (SQ counters of the same workload).
-    busy = profile_figure("batched_step_mixed512_sq_counters.md", f"valu_busy[{name}]")
+    sec = ms_launch / 1e3
+    hbm = per_launch / sec / 1e9 if sec > 0 else 0.0
+    hbm_model = {"achieved": round(hbm, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(hbm / HBM_PEAK_GBS, 4), "alg_bytes_per_launch": round(per_launch)}
+    out = {"kernel": name, "kernel_ms_per_launch": round(ms_launch, 3), "launches_per_step": launches_step,
+           "traffic": profiled_traffic(name)}
+    insts = profile_figure(SQ_PROFILE, f"valu_insts[{name}]")
+    lanes = profile_figure(SQ_PROFILE, f"lanes_active[{name}]")
+    busy = profile_figure(SQ_PROFILE, f"valu_busy[{name}]")
+    atomics = profile_figure(HBM_PROFILE, f"global_atomics_per_launch[{name}]")
+    if bound == "valu" and insts is not None and lanes is not None and sec > 0:
+        achieved = insts * lanes / sec / 1e12
+        out.update({"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_LANE_PEAK / 1e12, 2),
+                    "unit": "T lane-ops/s", "frac": round(achieved / (VALU_LANE_PEAK / 1e12), 4),
+                    "valu_insts_per_launch": insts, "hbm_model": hbm_model})
+    else:
+        out.update({"bound": "hbm", "achieved": hbm_model["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm_model["frac"], "alg_bytes_per_launch": hbm_model["alg_bytes_per_launch"]})
     if busy is not None:
         out["valu_busy"] = busy
-    # and how many of a wave's 64 lanes take part in those instructions
-    lanes = profile_figure("batched_step_mixed512_sq_counters.md", f"lanes_active[{name}]")
     if lanes is not None:
         out["lanes_active"] = lanes
     if busy is not None and lanes is not None:
         # the share of the SIMDs' lane slots doing useful VALU work
         out["valu_lane_frac"] = round(busy * lanes / 64.0, 3)
+    if atomics is not None and sec > 0:
+        out["global_atomics_per_launch"] = atomics
+        out["atomics_gops"] = round(atomics / sec / 1e9, 3)
     return out
 
 
@@ -438,6 +523,19 @@ def main() -> int:
     value = total / ms_step / 1e3
     log(f"batched (graph replay): {ms_step:.2f} ms/step ({wall_ms:.2f} ms wall) -> {value:.1f} {UNIT}")
     plan.close()
+    # the two job classes of the stream apart: each half of the table as its
+    # own captured schedule (camera renders alone, volume renders alone)
+    split = {"mpaths_s_in_mix": round(paths / ms_step / 1e3, 1),
+             "mphotons_s_in_mix": round(photons / ms_step / 1e3, 1)}
+    for label, kind, units in (("camera_only", A.JOB_CAMERA, paths), ("volume_only", A.JOB_VOLUME, photons)):
+        sub = [j for j, s in enumerate(specs) if s.kind == kind]
+        p = br.plan(stream, subset=sub)
+        p.launch(stream)
+        ms_sub, _ = timer.run(lambda: p.launch(stream), args.steps)
+        p.close()
+        split[f"{label}_ms_per_step"] = round(ms_sub, 3)
+        split[f"{label}_{'mpaths_s' if kind == A.JOB_CAMERA else 'mphotons_s'}"] = round(units / ms_sub / 1e3, 1)
+    log(f"classes apart: {split}")
     # the same schedule launched eagerly (no graph)
     br.run_batched(stream)
     ctx.sync(stream)
@@ -474,28 +572,27 @@ def main() -> int:
                                   kt["photon"][1] // steps),
     }
     if kt["bins"][1]:
-        groups["bins"] = kernel_roofline("k_bins passes", dep_bytes, kt["bins"][0] / steps, kt["bins"][1] // steps)
-    # The second ceiling: the deposits as fp32 atomics (3 per deposit) over
-    # the photon kernel's time, against the measured atomic peak.
+        groups["bins"] = kernel_roofline("k_bins passes", dep_bytes, kt["bins"][0] / steps, kt["bins"][1] // steps,
+                                         bound="hbm")
+    # scattered fp32 atomic peak: the ceiling of the global atomics each
+    # group issues (counted by rocprofv3, so a fraction cannot exceed 1)
     a_peak = atomic_peak(ctx, timer)
-    ph_ms = groups["photon"]["kernel_ms_per_launch"] * groups["photon"]["launches_per_step"]
-    dep_atomics = photons * wm["splat_per_photon"] * 3
     for g in groups.values():
-        g["atomic_peak_gops"] = round(a_peak, 2)
-    groups["photon"]["atomics_gops"] = round(dep_atomics / (ph_ms / 1e3) / 1e9, 2)
-    groups["photon"]["atomic_frac"] = round(groups["photon"]["atomics_gops"] / a_peak, 3)
+        if "atomics_gops" in g:
+            g["atomic_frac"] = round(g["atomics_gops"] / a_peak, 4)
     dominant = max(groups, key=lambda g: groups[g]["kernel_ms_per_launch"] * groups[g]["launches_per_step"])
     roofline = dict(groups[dominant])
-    roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "kernel_ms_per_launch", "launches_per_step", "achieved",
-                                                 "frac", "traffic", "valu_busy", "lanes_active", "valu_lane_frac",
-                                                 "atomics_gops", "atomic_frac")
+    roofline["kernels"] = {g: {k: v[k] for k in ("kernel", "bound", "kernel_ms_per_launch", "launches_per_step",
+                                                 "achieved", "unit", "frac", "traffic", "valu_busy", "lanes_active",
+                                                 "valu_lane_frac", "hbm_model", "atomics_gops", "atomic_frac")
                                if k in v}
                            for g, v in groups.items()}
     roofline["atomic_peak_gops"] = round(a_peak, 2)
-    roofline["note"] = ("achieved = algorithmic bytes (wavefront path state + deposits) / kernel time; the "
-                        "persistent kernels keep path state in registers, so measured HBM traffic is the "
-                        "smaller `traffic` and the bound is VALU issue (`valu_busy`), with `lanes_active` of a "
-                        "wave's 64 lanes working on average")
+    roofline["note"] = ("the persistent trace kernels keep path state in registers and issue VALU work on most "
+                        "SIMD cycles: achieved = VALU lane operations per launch (SQ_INSTS_VALU x lanes active, "
+                        "rocprofv3 counters of this workload) / the launch time measured here, against the VALU "
+                        "lane peak; `hbm_model` is the wavefront state model's bytes over the same time and "
+                        "`traffic` the counted HBM bytes per launch")
     roofline["work_model"] = {k: round(v, 4) for k, v in wm.items() if k != "configs"}
     if "configs" in wm:
         roofline["work_model_configs"] = wm["configs"]
@@ -513,6 +610,7 @@ def main() -> int:
         "eager_batched_ms_per_step": round(eager_ms, 3),
         "sequential_batched_ms_per_step": round(seq_ms, 3),
         "tuning": dict(tuning, **tune_info),
+        "mixed_classes": split,
         "roofline": roofline,
     }
     if not args.no_configs:

@@ -138,6 +138,8 @@ class StreamResult:
     units: int
     steps: int
     ids: list
+    paths: int = 0     # camera paths traced (all frames)
+    photons: int = 0   # photons traced (all frames)
 
 
 def run_stream(srv: Server, reqs: List[tuple], rate: float, seed: int = 0, stream=None,
@@ -171,12 +173,15 @@ def run_stream(srv: Server, reqs: List[tuple], rate: float, seed: int = 0, strea
                 if not keep:
                     srv.release(rid)
     wall = srv.clock() - t0
-    units = sum(s.units * k for s, k in reqs)
-    return StreamResult(lat, wall, units, steps, ids)
+    paths = sum(s.units * k for s, k in reqs if s.kind == A.JOB_CAMERA)
+    photons = sum(s.units * k for s, k in reqs if s.kind != A.JOB_CAMERA)
+    return StreamResult(lat, wall, paths + photons, steps, ids, paths, photons)
 
 
-def bench_serve(ctx: Context, quick: bool = False, n_req: int = 48) -> dict:
-    """Saturation throughput, then p50/p99 latency at 0.25x .. 2x of it."""
+def bench_serve(ctx: Context, quick: bool = False, n_req: int = 400) -> dict:
+    """Saturation throughput, then p50/p99 latency at 0.25x .. 2x of it, over
+    n_req requests per load point (p99 is then the 4th-largest latency of 400,
+    not the maximum); paths and photons per second are reported apart."""
     kw = dict(max_side=512, max_photons_log2=22) if quick else {}
     srv = Server(ctx)
     try:
@@ -185,6 +190,8 @@ def bench_serve(ctx: Context, quick: bool = False, n_req: int = 48) -> dict:
         sat_rate = len(sat_reqs) / res.wall_s
         out = {"saturation_req_s": round(sat_rate, 3),
                "saturation_value": round(res.units / res.wall_s / 1e6, 1),
+               "saturation_mpaths_s": round(res.paths / res.wall_s / 1e6, 1),
+               "saturation_mphotons_s": round(res.photons / res.wall_s / 1e6, 1),
                "requests_per_point": n_req, "points": []}
         for mult in (0.25, 0.5, 1.0, 2.0):
             reqs = request_mix(n_req, 11, **kw)
@@ -193,7 +200,9 @@ def bench_serve(ctx: Context, quick: bool = False, n_req: int = 48) -> dict:
                 "load": mult, "rate_req_s": round(mult * sat_rate, 3),
                 "p50_ms": round(float(np.nanpercentile(r.latencies, 50)) * 1e3, 2),
                 "p99_ms": round(float(np.nanpercentile(r.latencies, 99)) * 1e3, 2),
-                "throughput": round(r.units / r.wall_s / 1e6, 1), "steps": r.steps})
+                "throughput": round(r.units / r.wall_s / 1e6, 1),
+                "mpaths_s": round(r.paths / r.wall_s / 1e6, 1), "mphotons_s": round(r.photons / r.wall_s / 1e6, 1),
+                "steps": r.steps})
         out["graph_steps"] = srv.stats()
         return out
     finally:
