@@ -6,28 +6,19 @@
 // -> bounce 1 .. max_depth-1 -> (camera) resolve.  A camera chunk holds
 // whole pixels (all spp samples of each), and the resolve kernel sums each
 // pixel's samples in sample order, so the camera film equals the plain
-// kernel's bit for bit.
+// kernel's bit for bit.  Camera paths are numbered sample-major (slot =
+// sample * pixels + pixel): neighbouring lanes trace neighbouring pixels,
+// and the resolve reads each sample's results coalesced.  A bounce kernel's
+// grid is sized before its queue length is known, so blocks past the queue
+// return before they stage the scene.
 #include "spx_paths.hpp"
 #include "spx_splat.hpp"
 
 #include <string.h>
 
-namespace spx {
+#include <algorithm>
 
-// Live-path compaction: one atomic per wave, slot = base + prefix(ballot).
-__device__ inline uint32_t wave_enqueue(bool alive, uint32_t *counter) {
-    uint64_t mask = __ballot(alive);
-    uint32_t lane = __lane_id();
-    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-    uint32_t total = (uint32_t)__popcll(mask);
-    uint32_t base = 0;
-    if (total) {
-        int leader = __ffsll((unsigned long long)mask) - 1;
-        if ((int)lane == leader) base = atomicAdd(counter, total);
-        base = __shfl(base, leader);
-    }
-    return base + prefix;
-}
+namespace spx {
 
 // ---------------------------------------------------------------------------
 // Camera queues: SoA {ox,oy,oz,dx,dy,dz,rng,u_hero,tp0,slot|collapsed<<31}.
@@ -67,10 +58,33 @@ __device__ inline uint32_t cam_load(const CamQueue &q, uint32_t i, CamState &st)
     return s & 0x7FFFFFFFu;
 }
 
+// Block-level compaction into a block-private queue region: one LDS atomic
+// per wave reserves the wave's slots, at = the slot within the region.
+__device__ inline uint32_t region_enqueue(bool alive, uint32_t *lds_count) {
+    uint64_t mask = __ballot(alive);
+    uint32_t lane = __lane_id();
+    uint32_t prefix = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t total = (uint32_t)__popcll(mask);
+    uint32_t base = 0;
+    if (total) {
+        int leader = __ffsll((unsigned long long)mask) - 1;
+        if ((int)lane == leader) base = atomicAdd(lds_count, total);
+        base = __shfl(base, leader);
+    }
+    return base + prefix;
+}
+
+// Camera chunks run on a fixed grid of G blocks; block g owns queue region
+// [g * seg, (g + 1) * seg) of both queues and keeps its paths there from
+// bounce to bounce (the survivors of a region never outnumber its paths),
+// so compaction needs no global atomic -- a counter every wave of the chip
+// would hit -- and a block's paths stay in its XCD's L2.  counts[b * G + g]
+// = paths of region g entering bounce b.
 struct CamWave {
     CamQueue q[2];
     float *rx, *ry, *rz;   // per-path XYZ results, chunk-local slot order
-    uint32_t *counts;      // [33] live counts per bounce
+    uint32_t *counts;      // [33 * G]
+    uint32_t G, seg;       // regions and their capacity
 };
 
 __device__ inline void cam_finish(const CamWave &W, const float4 *tbl, const CamState &st, uint32_t slot,
@@ -81,45 +95,76 @@ __device__ inline void cam_finish(const CamWave &W, const float4 *tbl, const Cam
     W.rz[slot] = c.z;
 }
 
-__global__ void k_reset_counts(uint32_t *counts, uint32_t n) {
-    if (threadIdx.x < n) counts[threadIdx.x] = 0;
-}
-
+// Emission + bounce 0 of paths [g * seg, (g + 1) * seg) of the chunk
+// (sample-major: path j is sample j / npix of chunk pixel j % npix).
 __global__ __launch_bounds__(256) void k_cam_first(CamParams P, CamWave W, uint32_t pix0, uint32_t npix) {
     __shared__ SceneData S;
     __shared__ float4 tbl[kTblN];
+    __shared__ uint32_t s_cnt;
     stage_scene(S, P.scene, tbl, P.tbl);
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const uint32_t total = npix * P.spp;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < total; j += stride) {
-        uint32_t pixel = pix0 + j / P.spp;
-        uint32_t k = j % P.spp;
+    const uint32_t j0 = blockIdx.x * W.seg;
+    const uint32_t j1 = min(j0 + W.seg, total);
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        uint32_t pixel = pix0 + j % npix;
+        uint32_t k = j / npix;
         CamState st;
         camera_begin(P, pixel % P.w, pixel / P.w, P.sample_base + k, st);
         bool alive = camera_bounce(S, tbl, P, 0, st) && P.max_depth > 1;
-        uint32_t at = wave_enqueue(alive, W.counts + 1);
-        if (alive) cam_store(W.q[1], at, st, j);
+        uint32_t at = region_enqueue(alive, &s_cnt);
+        if (alive) cam_store(W.q[1], j0 + at, st, j);
         else cam_finish(W, tbl, st, j, P.illuminant == SPX_ILLUM_E);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) W.counts[1u * W.G + blockIdx.x] = s_cnt;
 }
 
+// Bounce b of region g's surviving paths.
 __global__ __launch_bounds__(256) void k_cam_bounce(CamParams P, CamWave W, uint32_t b) {
     __shared__ SceneData S;
     __shared__ float4 tbl[kTblN];
+    __shared__ uint32_t s_cnt;
+    const uint32_t n_in = W.counts[b * W.G + blockIdx.x];
+    if (n_in == 0) return;  // count of the next bounce stays 0 (reset per chunk)
     stage_scene(S, P.scene, tbl, P.tbl);
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    const uint32_t n_in = W.counts[b];
     const CamQueue &qin = W.q[b & 1u];
     const CamQueue &qout = W.q[(b + 1u) & 1u];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += stride) {
+    const uint32_t r0 = blockIdx.x * W.seg;
+    for (uint32_t i = threadIdx.x; i < n_in; i += blockDim.x) {
         CamState st;
-        uint32_t slot = cam_load(qin, i, st);
+        uint32_t slot = cam_load(qin, r0 + i, st);
         bool alive = camera_bounce(S, tbl, P, b, st) && (b + 1u < P.max_depth);
-        uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
-        if (alive) cam_store(qout, at, st, slot);
+        uint32_t at = region_enqueue(alive, &s_cnt);
+        if (alive) cam_store(qout, r0 + at, st, slot);
         else cam_finish(W, tbl, st, slot, P.illuminant == SPX_ILLUM_E);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) W.counts[(b + 1u) * W.G + blockIdx.x] = s_cnt;
+}
+
+// The tail: region g's paths entering bounce b, each traced to its end in
+// one thread.  Past the first bounces few paths are left in a chunk, and a
+// launch per remaining bounce would cost more in launches and empty blocks
+// than compaction saves; the bounce sequence of every path is unchanged.
+__global__ __launch_bounds__(256) void k_cam_tail(CamParams P, CamWave W, uint32_t b0) {
+    __shared__ SceneData S;
+    __shared__ float4 tbl[kTblN];
+    const uint32_t n_in = W.counts[b0 * W.G + blockIdx.x];
+    if (n_in == 0) return;
+    stage_scene(S, P.scene, tbl, P.tbl);
+    __syncthreads();
+    const CamQueue &qin = W.q[b0 & 1u];
+    const uint32_t r0 = blockIdx.x * W.seg;
+    for (uint32_t i = threadIdx.x; i < n_in; i += blockDim.x) {
+        CamState st;
+        uint32_t slot = cam_load(qin, r0 + i, st);
+        for (uint32_t b = b0; b < P.max_depth; ++b)
+            if (!camera_bounce(S, tbl, P, b, st)) break;
+        cam_finish(W, tbl, st, slot, P.illuminant == SPX_ILLUM_E);
     }
 }
 
@@ -128,11 +173,11 @@ __global__ __launch_bounds__(256) void k_cam_resolve(CamParams P, CamWave W, uin
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= npix) return;
     float ax = 0.f, ay = 0.f, az = 0.f;
-    const uint32_t base = j * P.spp;
-    for (uint32_t k = 0; k < P.spp; ++k) {
-        ax += W.rx[base + k];
-        ay += W.ry[base + k];
-        az += W.rz[base + k];
+    for (uint32_t k = 0; k < P.spp; ++k) {  // sample order; sample-major slots
+        const size_t at = (size_t)k * npix + j;
+        ax += W.rx[at];
+        ay += W.ry[at];
+        az += W.rz[at];
     }
     uint32_t i = pix0 + j;
     P.film[3 * i + 0] += ax;
@@ -142,16 +187,24 @@ __global__ __launch_bounds__(256) void k_cam_resolve(CamParams P, CamWave W, uin
 
 static size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Regions: at most kCamRegions blocks, each region a multiple of 256 paths
+// (queues sized chunk + kCamRegions * 256 to hold the rounding).
+constexpr uint32_t kCamRegions = 2048;
+constexpr uint32_t kMaxBounces = 33;
+// Bounces from this one on run in one tail launch (k_cam_tail).
+constexpr uint32_t kCamTailFrom = 3;
+
 size_t camera_wavefront_scratch(uint32_t chunk) {
-    return 2 * 10 * align_up((size_t)chunk * 4) + 3 * align_up((size_t)chunk * 4) + 256;
+    const size_t cap = (size_t)chunk + (size_t)kCamRegions * 256;
+    return align_up((size_t)kMaxBounces * kCamRegions * 4) + 2 * 10 * align_up(cap * 4) + 3 * align_up(cap * 4);
 }
 
 static CamWave carve_cam(void *scratch, uint32_t chunk) {
     char *p = (char *)scratch;
-    size_t a = align_up((size_t)chunk * 4);
+    const size_t a = align_up(((size_t)chunk + (size_t)kCamRegions * 256) * 4);
     CamWave W;
     W.counts = (uint32_t *)p;
-    p += 256;
+    p += align_up((size_t)kMaxBounces * kCamRegions * 4);
     for (int k = 0; k < 2; ++k) {
         CamQueue &q = W.q[k];
         q.ox = (float *)p; p += a;
@@ -168,6 +221,8 @@ static CamWave carve_cam(void *scratch, uint32_t chunk) {
     W.rx = (float *)p; p += a;
     W.ry = (float *)p; p += a;
     W.rz = (float *)p;
+    W.G = 0;
+    W.seg = 0;
     return W;
 }
 
@@ -183,15 +238,19 @@ hipError_t run_camera_wavefront(spx_ctx *ctx, const CamParams &p, void *scratch,
     uint32_t npix_total = p.w * p.h;
     uint32_t per = chunk / p.spp;
     if (per == 0) per = 1;  // spp > chunk: a chunk holds one pixel (needs chunk >= spp)
-    if ((uint64_t)per * p.spp > chunk) return hipErrorInvalidValue;
+    if ((uint64_t)per * p.spp > chunk || p.max_depth + 1u > kMaxBounces) return hipErrorInvalidValue;
     for (uint32_t pix0 = 0; pix0 < npix_total; pix0 += per) {
         uint32_t npix = npix_total - pix0 < per ? npix_total - pix0 : per;
         uint32_t total = npix * p.spp;
-        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
-        uint32_t g = persistent_blocks(ctx, total, 8);
-        hipLaunchKernelGGL(k_cam_first, dim3(g), dim3(256), 0, s, p, W, pix0, npix);
-        for (uint32_t b = 1; b < p.max_depth; ++b)
-            hipLaunchKernelGGL(k_cam_bounce, dim3(g), dim3(256), 0, s, p, W, b);
+        W.G = std::min(persistent_blocks(ctx, total, 8), kCamRegions);
+        W.seg = (((total + W.G - 1) / W.G) + 255u) & ~255u;  // G * seg <= chunk + G * 256
+        hipError_t e = launch_fill(reinterpret_cast<float *>(W.counts), 0.0f, (size_t)(p.max_depth + 1u) * W.G, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_cam_first, dim3(W.G), dim3(256), 0, s, p, W, pix0, npix);
+        const uint32_t tail = std::min(p.max_depth, kCamTailFrom);
+        for (uint32_t b = 1; b < tail; ++b)
+            hipLaunchKernelGGL(k_cam_bounce, dim3(W.G), dim3(256), 0, s, p, W, b);
+        if (tail < p.max_depth) hipLaunchKernelGGL(k_cam_tail, dim3(W.G), dim3(256), 0, s, p, W, tail);
         hipLaunchKernelGGL(k_cam_resolve, dim3((npix + 255) / 256), dim3(256), 0, s, p, W, pix0, npix);
     }
     return hipGetLastError();
@@ -224,9 +283,12 @@ __device__ inline void ph_load(const PhQueue &q, uint32_t i, const float4 *tbl, 
     st.cmf = cmf_lookup(tbl, st.lambda);
 }
 
+// Photon chunks use block-owned queue regions as camera chunks do:
+// counts[b * G + g] = photons of region g entering bounce b.
 struct PhWave {
     PhQueue q[2];
-    uint32_t *counts;
+    uint32_t *counts;  // [33 * G]
+    uint32_t G, seg;
 };
 
 constexpr int kSplatSlots = 2048;
@@ -265,26 +327,28 @@ struct PhSink<3> {
     uint32_t wstate[2 * 4];
 };
 
+// Bounce b of region g = blockIdx.x: the region's n_in photons (bounce 0:
+// chunk photons [g * seg, (g + 1) * seg)), survivors compacted into the
+// same region of the other queue.
 template <int MODE, int SINK>
 __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const PhotonParams &P, PhWave &W,
-                               uint32_t b, uint32_t start, uint32_t count, const BinState &bins,
-                               PhSink<SINK> &L) {
+                               uint32_t b, uint32_t start, uint32_t n_in, const BinState &bins,
+                               PhSink<SINK> &L, uint32_t *s_cnt) {
     const bool first = b == 0;
-    const uint32_t n_in = first ? count : W.counts[b];
     const PhQueue &qin = W.q[b & 1u];
     const PhQueue &qout = W.q[(b + 1u) & 1u];
-    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t r0 = blockIdx.x * W.seg;
     auto step = [&](auto &sink) {
         // Block-uniform trip count so that the flush barriers line up.
-        for (uint32_t base = blockIdx.x * blockDim.x; base < n_in; base += stride) {
+        for (uint32_t base = 0; base < n_in; base += blockDim.x) {
             uint32_t i = base + threadIdx.x;
             if (i < n_in) {
                 PhotonState st;
-                if (first) photon_begin(P, tbl, P.photon_base + start + i, st);
-                else ph_load(qin, i, tbl, st);
+                if (first) photon_begin(P, tbl, P.photon_base + start + r0 + i, st);
+                else ph_load(qin, r0 + i, tbl, st);
                 bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
-                uint32_t at = wave_enqueue(alive, W.counts + b + 1u);
-                if (alive) ph_store(qout, at, st);
+                uint32_t at = region_enqueue(alive, s_cnt);
+                if (alive) ph_store(qout, r0 + at, st);
             }
             if constexpr (SINK == 1 || SINK == 3) sink.maybe_flush();
         }
@@ -324,19 +388,35 @@ __global__ __launch_bounds__(256) void k_ph(PhotonParams P, PhWave W, uint32_t b
     __shared__ SceneData S;
     __shared__ float4 tbl[kTblN];
     __shared__ PhSink<SINK> L;
+    __shared__ uint32_t s_cnt;
+    const uint32_t r0 = blockIdx.x * W.seg;
+    const uint32_t n_in = b == 0 ? (r0 < count ? min(W.seg, count - r0) : 0u) : W.counts[b * W.G + blockIdx.x];
+    // an empty region returns at once (no barrier, flush or record chunk of
+    // its block is pending; its next count stays 0)
+    if (n_in == 0) return;
     stage_scene(S, P.scene, tbl, P.tbl);
+    if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
-    ph_body<MODE, SINK>(S, tbl, P, W, b, start, count, bins, L);
+    ph_body<MODE, SINK>(S, tbl, P, W, b, start, n_in, bins, L, &s_cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) W.counts[(b + 1u) * W.G + blockIdx.x] = s_cnt;
 }
 
-size_t photon_wavefront_scratch(uint32_t chunk) { return 2 * 9 * align_up((size_t)chunk * 4) + 256; }
+constexpr uint32_t kPhRegions = 1024;
+
+size_t photon_wavefront_scratch(uint32_t chunk) {
+    const size_t cap = (size_t)chunk + (size_t)kPhRegions * 256;
+    return align_up((size_t)kMaxBounces * kPhRegions * 4) + 2 * 9 * align_up(cap * 4);
+}
 
 static PhWave carve_ph(void *scratch, uint32_t chunk) {
     char *p = (char *)scratch;
-    size_t a = align_up((size_t)chunk * 4);
+    const size_t a = align_up(((size_t)chunk + (size_t)kPhRegions * 256) * 4);
     PhWave W;
     W.counts = (uint32_t *)p;
-    p += 256;
+    W.G = 0;
+    W.seg = 0;
+    p += align_up((size_t)kMaxBounces * kPhRegions * 4);
     for (int k = 0; k < 2; ++k) {
         PhQueue &q = W.q[k];
         q.ox = (float *)p; p += a;
@@ -360,11 +440,12 @@ static void run_ph(spx_ctx *ctx, const PhotonParams &p, PhWave W, uint32_t chunk
     const BinState &B = bins ? *bins : none;
     for (uint32_t start = 0; start < p.n; start += chunk) {
         uint32_t count = p.n - start < chunk ? p.n - start : chunk;
-        hipLaunchKernelGGL(k_reset_counts, dim3(1), dim3(64), 0, s, W.counts, 33u);
+        W.G = std::min(persistent_blocks(ctx, count, 4), kPhRegions);
+        W.seg = (((count + W.G - 1) / W.G) + 255u) & ~255u;  // G * seg <= chunk + G * 256
+        launch_fill(reinterpret_cast<float *>(W.counts), 0.0f, (size_t)(p.max_depth + 1u) * W.G, s);
         if (SINK >= 2) launch_bins_reset(B, s);
-        uint32_t g = persistent_blocks(ctx, count, 4);
         for (uint32_t b = 0; b < p.max_depth; ++b)
-            hipLaunchKernelGGL((k_ph<MODE, SINK>), dim3(g), dim3(256), 0, s, p, W, b, start, count, B);
+            hipLaunchKernelGGL((k_ph<MODE, SINK>), dim3(W.G), dim3(256), 0, s, p, W, b, start, count, B);
         if (SINK >= 2) launch_bins_resolve(B, job_shared, (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4, s);
     }
 }

@@ -658,7 +658,11 @@ int spx_camera_accumulate(spx_ctx *ctx, const spx_scene *scene, const spx_camera
     if (engine == SPX_ENGINE_PLAIN) {
         e = launch_camera_plain(p, s);
     } else if (engine == SPX_ENGINE_WAVEFRONT) {
-        uint32_t chunk = spp > (1u << 21) ? spp : (1u << 21);
+        // up to 2^24 paths per chunk (two queues of 10 words per path and
+        // 12 B of results: 1.5 GB), at least one pixel's samples
+        const uint64_t paths = (uint64_t)width * height * spp;
+        uint32_t chunk = (uint32_t)std::min<uint64_t>(paths, 1u << 24);
+        chunk = chunk > spp ? chunk : spp;
         if ((rc = ensure_scratch(ctx, camera_wavefront_scratch(chunk), s))) return rc;
         e = run_camera_wavefront(ctx, p, ctx->scratch, chunk, s);
     } else {
