@@ -291,11 +291,11 @@ def per_config(ctx, timer, quick):
         tbl = (A.Job * 1)(job)
         return lambda: _check(ctx, A.lib().spx_batch_run(ctx.handle, tbl, 1, None, st))
 
-    def volume_job(ds, cam, bm, w, h, n, ill, disp, film, zbuf=None):
+    def volume_job(ds, cam, bm, w, h, n, ill, disp, film, zbuf=None, hero=0):
         j = A.Job()
         j.kind, j.width, j.height, j.n_photons, j.seed = A.JOB_VOLUME, w, h, n, 0
         j.estimator = A.EST_POINT
-        j.opts = opts(ill, disp)
+        j.opts = opts(ill, disp, hero_lanes=hero)
         j.camera, j.beam, j.vol = cam.basis, bm, A.VolParams(**scenes.HAZE)
         j.scene, j.film = ds.handle.value, film.ptr
         j.zbuf = zbuf.ptr if zbuf is not None else None
@@ -310,6 +310,13 @@ def per_config(ctx, timer, quick):
                                                    stream=st)))
              for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront"))]
     forms.append(("batched", batched(volume_job(sc, cam, bm, 320, 180, 1 << 20, A.ILLUM_D65, True, film))))
+    # the same with 4 stratified hero wavelengths per photon (hero_lanes = 4)
+    forms += [(f"hero4_{nm}", (lambda eng=eng: S.render_volume(ctx, sc, cam, bm, 320, 180, 1 << 20, 0.05, 0.05, 0.0,
+                                                              14.0, None, 0, film=film, illuminant=A.ILLUM_D65,
+                                                              engine=eng, stream=st, hero_lanes=4)))
+              for eng, nm in ((A.ENGINE_PLAIN, "plain"), (A.ENGINE_WAVEFRONT, "wavefront"))]
+    forms.append(("hero4_batched", batched(volume_job(sc, cam, bm, 320, 180, 1 << 20, A.ILLUM_D65, True, film,
+                                                      hero=4))))
     for nm, f in forms:
         f()
         ms, _ = timer.run(f, 3)

@@ -132,6 +132,11 @@ typedef struct {
     uint32_t dispersion; /* 1 = n(lambda); 0 = n(550 nm) (vol_photons.rs:77 `spectral`) */
     uint32_t max_depth;  /* bounces (tracer.rs:39, lighttrace.rs:133, volume.rs:126): 1..8 */
     uint32_t rr_depth;   /* camera: 0 = no roulette; k = Russian roulette from bounce k */
+    uint32_t hero_lanes; /* photons: 0 or 1 = one wavelength per photon (the reference);
+                            4 = the stratified hero comb (spectrum.rs:20-32) per photon, its
+                            companion lanes collapsing at the first dielectric as a camera
+                            path's do (tracer.rs:86-91); not with SPX_EST_BEAM.  Camera
+                            paths always carry the comb. */
 } spx_opts;
 
 /* Volume weights (volume.rs:99-112; VolWeights, vol_photons.rs:256-267). */

@@ -730,11 +730,37 @@ static int screen_coords(const screen_t *S, v3 o, v3 d, float *s, float *r, floa
     return 0;
 }
 
-static void screen_splat(const screen_t *S, float *sum, float s, float r, float lambda,
-                         float power) {
-    /* Screen::texel_bilinear + splat, lighttrace.rs:65-91 */
+/* The photon spectra of the hero-comb option (hero_lanes = 4; an extension,
+ * no reference counterpart): a photon carries the stratified comb of
+ * spectrum.rs:20-32 drawn from its wavelength draw u, lane k weighing 1/4,
+ * until it scatters off a dielectric, where the companions collapse onto
+ * the hero lane (as a camera path's do, tracer.rs:86-91).  Its deposits are
+ * spec * (material weights * geometry), spec = sum_k cmf(lambda_k) * s_k *
+ * 0.25 (k in order) before the collapse and cmf(lambda_0) * s_0 after, s_k
+ * the beam's SPD weight (1 for equal energy). */
+static void comb_spec(float u, uint32_t ill, float ill_scale, float spec[3]) {
+    float lam[4];
+    or_hero_lambdas(u, lam);
+    spec[0] = spec[1] = spec[2] = 0.0f;
+    for (int k = 0; k < 4; ++k) {
+        float c[3];
+        or_cmf(lam[k], c);
+        float s = ill <= 1u ? or_illuminant(ill, lam[k]) * ill_scale : 1.0f;
+        for (int i = 0; i < 3; ++i) spec[i] += c[i] * s * 0.25f;
+    }
+}
+
+static void hero_spec(float lambda0, uint32_t ill, float ill_scale, float spec[3]) {
     float c[3];
-    or_cmf(lambda, c);
+    or_cmf(lambda0, c);
+    float s = ill <= 1u ? or_illuminant(ill, lambda0) * ill_scale : 1.0f;
+    for (int i = 0; i < 3; ++i) spec[i] = c[i] * s;
+}
+
+static void screen_splat(const screen_t *S, float *sum, float s, float r, const float c[3],
+                         float power) {
+    /* Screen::texel_bilinear + splat, lighttrace.rs:65-91 (c = cmf(lambda), or
+     * the photon's comb spectrum) */
     float fx = clampf(s * (float)S->w - 0.5f, 0.0f, (float)S->w - 1.0f);
     float fy = clampf((1.0f - r) * (float)S->h - 0.5f, 0.0f, (float)S->h - 1.0f);
     uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
@@ -751,10 +777,12 @@ static void screen_splat(const screen_t *S, float *sum, float s, float r, float 
     }
 }
 
-/* trace_photon, lighttrace.rs:125-153.  Returns 1 if it landed. */
+/* trace_photon, lighttrace.rs:125-153.  Returns 1 if it landed.  comb (may
+ * be NULL): set while the photon carries the hero comb, cleared at its
+ * first dielectric scatter. */
 static int trace_photon(const or_scene *sc, const screen_t *S, v3 o, v3 d, float lambda,
                         float power0, uint32_t *rng, const or_opts *op, float *s, float *r,
-                        float *power_out, uint64_t *segs) {
+                        float *power_out, uint64_t *segs, int *comb) {
     float power = power0;
     for (uint32_t b = 0; b < op->max_depth; ++b) {
         if (segs) ++*segs;
@@ -778,6 +806,7 @@ static int trace_photon(const or_scene *sc, const screen_t *S, v3 o, v3 d, float
         int tir;
         scatter(m, d, &hit, n_hero, rng, &nd, &wgt, &tir);
         power *= wgt;
+        if (comb && m->kind == 1u) *comb = 0;
         o = hit.point;
         d = nd;
     }
@@ -788,7 +817,7 @@ static int trace_photon(const or_scene *sc, const screen_t *S, v3 o, v3 d, float
 int or_lighttrace_photon(const or_scene *sc, const or_screen *scr, const float o[3], const float d[3],
                          float lambda, uint32_t *rng, const or_opts *op, float out[3]) {
     screen_t S = screen_make(scr);
-    return trace_photon(sc, &S, vl(o), vl(d), lambda, 1.0f, rng, op, &out[0], &out[1], &out[2], NULL);
+    return trace_photon(sc, &S, vl(o), vl(d), lambda, 1.0f, rng, op, &out[0], &out[1], &out[2], NULL, NULL);
 }
 
 typedef struct {
@@ -811,12 +840,21 @@ static void *lt_worker(void *arg) {
         float a = next_f32(&rng);
         float b = next_f32(&rng);
         v3 origin = add(add(bc, mul(bu, a)), mul(bv, b));
-        float lambda = LAMBDA_MIN + LAMBDA_RANGE * next_f32(&rng);
-        float power0 = j->o->illuminant <= 1u ? or_illuminant(j->o->illuminant, lambda) * j->ill_scale
-                                              : 1.0f;
+        float u = next_f32(&rng);
+        float lambda = LAMBDA_MIN + LAMBDA_RANGE * u;
+        const int hero = j->o->hero_lanes == 4u;
+        float power0 = hero ? 1.0f
+                            : (j->o->illuminant <= 1u ? or_illuminant(j->o->illuminant, lambda) * j->ill_scale
+                                                      : 1.0f);
         float s, r, p;
-        if (trace_photon(j->sc, j->S, origin, bd, lambda, power0, &rng, j->o, &s, &r, &p, &j->segs)) {
-            screen_splat(j->S, j->sum, s, r, lambda, p);
+        int comb = hero;
+        if (trace_photon(j->sc, j->S, origin, bd, lambda, power0, &rng, j->o, &s, &r, &p, &j->segs,
+                         hero ? &comb : NULL)) {
+            float c[3];
+            if (!hero) or_cmf(lambda, c);
+            else if (comb) comb_spec(u, j->o->illuminant, j->ill_scale, c);
+            else hero_spec(lambda, j->o->illuminant, j->ill_scale, c);
+            screen_splat(j->S, j->sum, s, r, c, p);
             ++j->splats;
         }
     }
@@ -986,12 +1024,19 @@ static void *vol_worker(void *arg) {
         float a = next_f32(&rng);
         float b = next_f32(&rng);
         v3 ro = add(add(bc, mul(bu, a)), mul(bv, b));
-        float lambda = LAMBDA_MIN + LAMBDA_RANGE * next_f32(&rng);
+        float u = next_f32(&rng);
+        float lambda = LAMBDA_MIN + LAMBDA_RANGE * u;
         float cmf[3];
-        or_cmf(lambda, cmf);
         v3 rd = bd;
-        float power = j->o->illuminant <= 1u ? or_illuminant(j->o->illuminant, lambda) * j->ill_scale
-                                             : 1.0f;
+        float power;
+        int comb = j->o->hero_lanes == 4u;  /* the hero comb's lanes still live */
+        if (comb) {
+            comb_spec(u, j->o->illuminant, j->ill_scale, cmf);  /* spectra in cmf, power = material weights */
+            power = 1.0f;
+        } else {
+            or_cmf(lambda, cmf);
+            power = j->o->illuminant <= 1u ? or_illuminant(j->o->illuminant, lambda) * j->ill_scale : 1.0f;
+        }
         for (uint32_t bnc = 0; bnc < j->o->max_depth; ++bnc) {
             ++j->segs;
             hit_t hit;
@@ -1092,6 +1137,10 @@ static void *vol_worker(void *arg) {
             int tir;
             scatter(mt, rd, &hit, n_hero, &rng, &nd, &wgt, &tir);
             power *= wgt;
+            if (comb && mt->kind == 1u) {  /* first dielectric: the comb collapses to the hero lane */
+                hero_spec(lambda, j->o->illuminant, j->ill_scale, cmf);
+                comb = 0;
+            }
             ro = hit.point;
             rd = nd;
         }

@@ -101,6 +101,7 @@ typedef struct {
     uint32_t dispersion;  /* 1 = n(lambda); 0 = n(550 nm) for every lambda */
     uint32_t max_depth;   /* bounce cap; the reference fixes 8 */
     uint32_t rr_depth;    /* camera only: 0 = off, k = roulette from bounce k */
+    uint32_t hero_lanes;  /* photons: 0/1 = one wavelength; 4 = the hero comb (extension) */
 } or_opts;
 
 /* Volume estimator parameters (volume.rs:99-112). */

@@ -65,7 +65,7 @@ class ScreenDesc(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("illuminant", C.c_uint32), ("dispersion", C.c_uint32),
-                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("hero_lanes", C.c_uint32)]
 
 
 class VolParams(C.Structure):
@@ -117,7 +117,7 @@ class RequestInfo(C.Structure):
 REQ_PENDING, REQ_ACTIVE, REQ_DONE = 0, 1, 2
 
 STRUCT_SIZES = {"prim": 32, "plane": 16, "material": 16, "env": 32, "camera": 48, "beam": 48,
-                "screen": 44, "opts": 16, "vol_params": 16, "photon_record": 304}
+                "screen": 44, "opts": 20, "vol_params": 16, "photon_record": 304}
 
 _vp, _u32, _f32, _i32, _sz = C.c_void_p, C.c_uint32, C.c_float, C.c_int, C.c_size_t
 _P = C.POINTER

@@ -300,8 +300,9 @@ def screen(corner, u, v, width: int, height: int) -> A.ScreenDesc:
 
 
 def opts(illuminant: int = A.ILLUM_D65, dispersion: bool = True, max_depth: int = 8,
-         rr_depth: int = 0) -> A.Opts:
-    return A.Opts(int(illuminant), 1 if dispersion else 0, int(max_depth), int(rr_depth))
+         rr_depth: int = 0, hero_lanes: int = 0) -> A.Opts:
+    """spx_opts; hero_lanes = 4 gives photons the stratified 4-wavelength comb."""
+    return A.Opts(int(illuminant), 1 if dispersion else 0, int(max_depth), int(rr_depth), int(hero_lanes))
 
 
 # ---------------------------------------------------------------------------
@@ -347,11 +348,12 @@ class HipTracer:
 
 def lighttrace_trace(ctx: Context, scene, scr: A.ScreenDesc, bm: A.BeamDesc, n: int, seed: int,
                      film: Optional[Film] = None, photon_base: int = 0, illuminant: int = A.ILLUM_E,
-                     dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None) -> Film:
+                     dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None,
+                     hero_lanes: int = 0) -> Film:
     """lighttrace::trace (lighttrace.rs:157-166): adds n photons into the screen film."""
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
     film = film or Film(ctx, scr.width, scr.height, stream)
-    o = opts(illuminant, dispersion)
+    o = opts(illuminant, dispersion, hero_lanes=hero_lanes)
     _check(ctx, A.lib().spx_lighttrace(ctx.handle, ds.handle, C.byref(scr), C.byref(bm), n,
                                        photon_base, seed, C.byref(o), engine, film.ptr, stream))
     return film
@@ -361,11 +363,12 @@ def render_volume(ctx: Context, scene, cam: Camera, bm: A.BeamDesc, width: int, 
                   n_photons: int, sigma_s: float, sigma_t: float, g: float, max_dist: float,
                   zbuffer: Optional[DeviceBuffer], seed: int, estimator: int = A.EST_POINT,
                   film: Optional[Film] = None, photon_base: int = 0, illuminant: int = A.ILLUM_E,
-                  dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None) -> Film:
+                  dispersion: bool = True, engine: int = A.ENGINE_PLAIN, stream=None,
+                  hero_lanes: int = 0) -> Film:
     ds = scene.upload(ctx) if isinstance(scene, Scene) else scene
     film = film or Film(ctx, width, height, stream)
     vp = A.VolParams(sigma_s, sigma_t, g, max_dist)
-    o = opts(illuminant, dispersion)
+    o = opts(illuminant, dispersion, hero_lanes=hero_lanes)
     _check(ctx, A.lib().spx_volume(ctx.handle, ds.handle, C.byref(cam.basis), C.byref(bm), width,
                                    height, n_photons, photon_base, C.byref(vp),
                                    zbuffer.ptr if zbuffer else None, seed, estimator, C.byref(o),

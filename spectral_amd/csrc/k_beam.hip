@@ -47,7 +47,7 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
         q[1] = make_float4(st.rd.x, st.rd.y, st.rd.z, st.power);
         q[2] = make_float4(st.lambda, __uint_as_float(m), 0.0f, 0.0f);
         if (mi < 0) break;
-        photon_scatter(S, P, mi, h, st);
+        photon_scatter(S, tbl, P, mi, h, st);
     }
 }
 

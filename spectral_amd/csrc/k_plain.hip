@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_photon_records(PhotonParams P, uint32_t
             f3 dummy;
             if (!refract(st.rd, h.normal, n1, n2, dummy)) rec.tir_count++;
         }
-        photon_scatter(S, P, mi, h, st);
+        photon_scatter(S, tbl, P, mi, h, st);
         if (ns < 9) {
             rec.states[ns][0][0] = st.ro.x; rec.states[ns][0][1] = st.ro.y; rec.states[ns][0][2] = st.ro.z;
             rec.states[ns][1][0] = st.rd.x; rec.states[ns][1][1] = st.rd.y; rec.states[ns][1][2] = st.rd.z;

@@ -265,22 +265,30 @@ struct PhQueue {
     float *lam, *pw;
 };
 
+// The wavelength travels as its draw u (lambda = 380 + 350 u, recomputed
+// bit for bit) and the power's sign bit marks a live hero comb (powers are
+// >= 0), so the spectrum of either option is rebuilt on load.
 __device__ inline void ph_store(const PhQueue &q, uint32_t i, const PhotonState &st) {
     q.ox[i] = st.ro.x; q.oy[i] = st.ro.y; q.oz[i] = st.ro.z;
     q.dx[i] = st.rd.x; q.dy[i] = st.rd.y; q.dz[i] = st.rd.z;
     q.rng[i] = st.rng;
-    q.lam[i] = st.lambda;
-    q.pw[i] = st.power;
+    q.lam[i] = st.u;
+    q.pw[i] = __uint_as_float(__float_as_uint(st.power) | (st.comb ? 0x80000000u : 0u));
 }
 
-__device__ inline void ph_load(const PhQueue &q, uint32_t i, const float4 *tbl, PhotonState &st) {
+__device__ inline void ph_load(const PhQueue &q, uint32_t i, const PhotonParams &P, const float4 *tbl,
+                               PhotonState &st) {
     st.ro = mk(q.ox[i], q.oy[i], q.oz[i]);
     st.rd = mk(q.dx[i], q.dy[i], q.dz[i]);
     st.rng = q.rng[i];
-    st.lambda = q.lam[i];
-    st.power = q.pw[i];
+    st.u = q.lam[i];
+    st.lambda = kLambdaMin + kLambdaRange * st.u;
+    const uint32_t pw = __float_as_uint(q.pw[i]);
+    st.power = __uint_as_float(pw & 0x7FFFFFFFu);
+    st.comb = pw >> 31;
     st.n_glass = kNoGlass;
-    st.cmf = cmf_lookup(tbl, st.lambda);
+    st.cmf = st.comb ? photon_comb(P, tbl, st.u) : (P.hero ? photon_hero(P, tbl, st.lambda)
+                                                            : cmf_lookup(tbl, st.lambda));
 }
 
 // Photon chunks use block-owned queue regions as camera chunks do:
@@ -345,7 +353,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
             if (i < n_in) {
                 PhotonState st;
                 if (first) photon_begin(P, tbl, P.photon_base + start + r0 + i, st);
-                else ph_load(qin, r0 + i, tbl, st);
+                else ph_load(qin, r0 + i, P, tbl, st);
                 bool alive = ph_bounce<MODE>(S, tbl, P, st, sink) && (b + 1u < P.max_depth);
                 uint32_t at = region_enqueue(alive, s_cnt);
                 if (alive) ph_store(qout, r0 + at, st);

@@ -320,6 +320,7 @@ bool spx::valid_opts(const spx_opts *o, bool camera) {
     if (o->max_depth < 1 || o->max_depth > 32) return false;
     if (o->illuminant > SPX_ILLUM_E) return false;
     if (o->dispersion > 1) return false;
+    if (o->hero_lanes != 0 && o->hero_lanes != 1 && o->hero_lanes != 4) return false;
     (void)camera;
     return true;
 }
@@ -330,6 +331,7 @@ spx_opts spx::default_opts(bool camera) {
     o.dispersion = 1;
     o.max_depth = 8;
     o.rr_depth = 0;
+    o.hero_lanes = 0;
     return o;
 }
 
@@ -359,6 +361,7 @@ int spx::photon_params(spx_ctx *ctx, const spx_scene *scene, const spx_beam *bea
     p.dispersion = o.dispersion;
     p.max_depth = o.max_depth;
     p.ill_scale = illuminant_scale(ctx, o.illuminant);
+    p.hero = o.hero_lanes == 4 ? 1u : 0u;
     p.replicas = 1;
     p.rep_stride = 0;
     p.scene = scene->dev;
@@ -725,6 +728,8 @@ int spx_volume(spx_ctx *ctx, const spx_scene *scene, const spx_camera *cam, cons
         return set_error(ctx, SPX_EINVAL, "spx_volume: bad volume parameters");
     PhotonParams p;
     if ((rc = photon_params(ctx, scene, beam, opts, n, photon_base, seed, p, "spx_volume"))) return rc;
+    if (p.hero && estimator == SPX_EST_BEAM)
+        return set_error(ctx, SPX_EINVAL, "spx_volume: the hero comb is not offered with the beam estimator");
     make_proj(*cam, p.proj);
     p.vp = *vp;
     p.w = width;

@@ -39,7 +39,7 @@ struct PhotonParams {
     uint32_t max_depth;
     uint32_t key_base;     // binned deposits: first record key of this job's film
     float ill_scale;       // photon power weight S(lambda)/mean S (1 for equal energy)
-    float pad5;
+    uint32_t hero;         // 1: photons carry the 4-wavelength hero comb (spx_opts.hero_lanes = 4)
     const SceneData *scene;
     const float4 *tbl;
     const float *zbuf;     // may be null (= +INF)

@@ -123,8 +123,39 @@ struct PhotonState {
     float power;
     float n;           // Glass::n of glass n_glass at this photon's wavelength
     uint32_t n_glass;  // kNoGlass until the first dielectric hit
-    f3 cmf;            // the CIE matching functions at this photon's wavelength
+    f3 cmf;            // the CIE matching functions at this photon's wavelength (the hero
+                       // comb: the photon's spectrum, see photon_comb)
+    uint32_t comb;     // the hero comb's companion lanes are still live
+    float u;           // the wavelength draw (lambda = 380 + 350 u)
 };
+
+// The photon spectra of the hero-comb option (P.hero; an extension with no
+// reference counterpart): the photon carries the stratified comb of
+// spectrum.rs:20-32 drawn from its wavelength draw, lane k weighing 1/4,
+// until it scatters off a dielectric, where the companions collapse onto
+// the hero lane (tracer.rs:86-91).  Deposits are spectrum * (material
+// weights * geometry): the spectrum is sum_k cmf(lambda_k) * s_k * 0.25 in
+// lane order before the collapse and cmf(lambda_0) * s_0 after, s_k the
+// beam's SPD weight (1 for equal energy), and the power starts at 1.
+__device__ inline f3 photon_comb(const PhotonParams &P, const float4 *tbl, float u) {
+    f3 spec = mk(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < kHeroN; ++k) {
+        const float lam = hero_lambda(u, k);
+        const f3 c = cmf_lookup(tbl, lam);
+        const float s = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, lam, false) * P.ill_scale;
+        spec.x += c.x * s * 0.25f;
+        spec.y += c.y * s * 0.25f;
+        spec.z += c.z * s * 0.25f;
+    }
+    return spec;
+}
+
+__device__ inline f3 photon_hero(const PhotonParams &P, const float4 *tbl, float lam) {
+    const f3 c = cmf_lookup(tbl, lam);
+    const float s = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, lam, false) * P.ill_scale;
+    return mk(c.x * s, c.y * s, c.z * s);
+}
 
 __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, uint32_t i,
                                     PhotonState &st) {
@@ -132,11 +163,19 @@ __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, ui
     float a = next_f32(st.rng);
     float b = next_f32(st.rng);
     st.ro = ld3(P.beam.corner) + ld3(P.beam.u) * a + ld3(P.beam.v) * b;
-    st.lambda = kLambdaMin + kLambdaRange * next_f32(st.rng);
+    const float u = next_f32(st.rng);
+    st.u = u;
+    st.lambda = kLambdaMin + kLambdaRange * u;  // = the comb's hero lane, hero_lambda(u, 0)
     st.rd = ld3(P.beam.dir);
-    st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
     st.n_glass = kNoGlass;
-    st.cmf = cmf_lookup(tbl, st.lambda);  // the wavelength is fixed: looked up once
+    st.comb = P.hero;
+    if (P.hero) {
+        st.power = 1.0f;
+        st.cmf = photon_comb(P, tbl, u);
+    } else {
+        st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
+        st.cmf = cmf_lookup(tbl, st.lambda);  // the wavelength is fixed: looked up once
+    }
 }
 
 // A deposit at film pixel (px, py) = index idx; sinks keyed by film tile
@@ -167,7 +206,7 @@ __device__ inline float photon_n(const spx_material &m, const PhotonParams &P, P
 }
 
 // Scatter at a hit (the tail of every photon loop).  Returns tir.
-__device__ inline bool photon_scatter(const SceneData &S, const PhotonParams &P, int mi, const Hit &h,
+__device__ inline bool photon_scatter(const SceneData &S, const float4 *tbl, const PhotonParams &P, int mi, const Hit &h,
                                       PhotonState &st) {
     const spx_material &m = S.mats[mi];
     float nh = photon_n(m, P, st);
@@ -175,6 +214,10 @@ __device__ inline bool photon_scatter(const SceneData &S, const PhotonParams &P,
     bool tir;
     float w = scatter(m, st.rd, h, nh, st.rng, nd, tir);
     st.power *= w;
+    if (st.comb && m.kind == SPX_MAT_DIELECTRIC) {  // the comb collapses onto the hero lane
+        st.cmf = photon_hero(P, tbl, st.lambda);
+        st.comb = 0;
+    }
     st.ro = h.point;
     st.rd = nd;
     return tir;
@@ -258,7 +301,7 @@ __device__ inline int lighttrace_bounce(const SceneData &S, const float4 *tbl, c
         return 1;
     }
     if (mi < 0) return 2;
-    photon_scatter(S, P, mi, h, st);
+    photon_scatter(S, tbl, P, mi, h, st);
     sink_scattered(sink, S.mats[mi].kind);
     return 0;
 }
@@ -473,7 +516,7 @@ __device__ inline bool volume_bounce(const SceneData &S, const float4 *tbl, cons
     else if (EST == SPX_EST_LEGACY) volume_point_segment<true>(P, tbl, st, cmf, seg_len, st.rng, sink);
     else volume_beam_segment(P, st, cmf, seg_len, sink);
     if (mi < 0) return false;
-    photon_scatter(S, P, mi, h, st);
+    photon_scatter(S, tbl, P, mi, h, st);
     sink_scattered(sink, S.mats[mi].kind);
     return true;
 }

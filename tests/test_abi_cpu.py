@@ -90,6 +90,6 @@ def test_batch_structs_match_header():
     for name, cls in (("spx_batch_cfg", A.BatchCfg), ("spx_job", A.Job)):
         names = [n for _, n in _header_fields(name)]
         assert names == [n for n, _ in cls._fields_], name
-    # spx_job: 10 u32 + opts 16 + camera 48 + beam 48 + screen 44 + vol 16 + 3 pointers
-    assert C.sizeof(A.Job) == 40 + 16 + 48 + 48 + 44 + 16 + 4 + 3 * 8
+    # spx_job: 10 u32 + opts 20 + camera 48 + beam 48 + screen 44 + vol 16 + 3 pointers
+    assert C.sizeof(A.Job) == 40 + 20 + 48 + 48 + 44 + 16 + 3 * 8
     assert C.sizeof(A.BatchCfg) == 16

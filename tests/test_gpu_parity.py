@@ -135,6 +135,52 @@ def test_prism_volume_film(ctx, oracle, engine, ill, disp):
     assert (err < FILM_TOL).all()
 
 
+def test_hero_comb_photon_films(ctx, oracle):
+    # photons carrying the 4-wavelength hero comb (hero_lanes = 4), the
+    # prism workload under D65 and A, plain / wavefront / batched, and the
+    # floor caustic's lighttrace: films against the oracle's comb
+    sc, bm, cam = scenes.prism_scene(), scenes.prism_beam(), scenes.prism_camera()
+    vp = A.VolParams(**scenes.HAZE)
+    n = 1 << 18
+    for ill in (A.ILLUM_D65, A.ILLUM_A):
+        c, _ = oracle.volume(sc, cam, bm, 320, 180, n, 0, vp, None, 0, A.EST_POINT, opts(ill, hero_lanes=4),
+                             threads=8)
+        one, _ = oracle.volume(sc, cam, bm, 320, 180, n, 0, vp, None, 0, A.EST_POINT, opts(ill), threads=8)
+        assert rel_l1_per_channel(c, one).max() > 1e-3  # the comb is a different estimate
+        for eng in (A.ENGINE_PLAIN, A.ENGINE_WAVEFRONT):
+            f = S.render_volume(ctx, sc, cam, bm, 320, 180, n, vp.sigma_s, vp.sigma_t, vp.g, vp.max_dist, None, 0,
+                                illuminant=ill, engine=eng, hero_lanes=4)
+            err = rel_l1_per_channel(f.image(), c)
+            f.free()
+            print(f"hero comb ill={ill} engine={eng}: rel L1 {err}")
+            assert (err < FILM_TOL).all() and (err < 1e-5).all()
+        ds = sc.upload(ctx)
+        film = S.Film(ctx, 320, 180)
+        j = A.Job()
+        j.kind, j.width, j.height, j.n_photons, j.seed = A.JOB_VOLUME, 320, 180, n, 0
+        j.estimator, j.opts = A.EST_POINT, opts(ill, hero_lanes=4)
+        j.camera, j.beam, j.vol = cam.basis, bm, vp
+        j.scene, j.film = ds.handle.value, film.ptr
+        assert A.lib().spx_batch_run(ctx.handle, (A.Job * 1)(j), 1, None, None) == 0
+        ctx.sync()
+        err = rel_l1_per_channel(film.image(), c)
+        print(f"hero comb ill={ill} batched: rel L1 {err}")
+        assert (err < 1e-5).all()
+        film.free()
+        ds.close()
+    scr = scenes.floor_screen(256, 256)
+    cc, _ = oracle.lighttrace(sc, scr, bm, 1 << 17, 0, 0x0F1, opts(A.ILLUM_D65, hero_lanes=4), threads=8)
+    for eng in (A.ENGINE_PLAIN, A.ENGINE_WAVEFRONT):
+        g = S.lighttrace_trace(ctx, sc, scr, bm, 1 << 17, 0x0F1, illuminant=A.ILLUM_D65, engine=eng, hero_lanes=4)
+        err = rel_l1_per_channel(g.image(), cc)
+        g.free()
+        print(f"hero comb lighttrace engine={eng}: rel L1 {err}")
+        assert (err < 1e-5).all()
+    with pytest.raises(A.SpxError):
+        S.render_volume(ctx, sc, cam, bm, 320, 180, 16, 0.05, 0.05, 0.0, 14.0, None, 0, estimator=A.EST_BEAM,
+                        hero_lanes=4)
+
+
 @pytest.mark.parametrize("engine", [A.ENGINE_PLAIN, A.ENGINE_WAVEFRONT])
 def test_floor_volume_film_with_zbuffer(ctx, oracle, engine):
     # prism + Lambertian floor, z-buffer from the surface pass (prism_on_plane.rs:125-164)

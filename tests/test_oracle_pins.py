@@ -99,9 +99,16 @@ def _hg(g, c):
     return (F(1) - g2) / (F(4) * F(math.pi) * den)
 
 
-def _point_film(scene_empty_beam, cam, w, h, n, seed, vp, power_of):
+def _comb(u):
+    """The stratified hero comb of spectrum.rs:20-32: 380 + rem_euclid(u 350 + k 87.5, 350)."""
+    return [F(380.0) + F(np.fmod(F(u) * F(350.0) + F(k) * F(87.5), F(350.0))) for k in range(4)]
+
+
+def _point_film(scene_empty_beam, cam, w, h, n, seed, vp, power_of, hero=False):
     """render_volumetric_scene_point (volume.rs:118-150) for an empty scene:
-    one escaped segment of length max_dist per photon, four samples."""
+    one escaped segment of length max_dist per photon, four samples.  hero:
+    the photon carries the 4-wavelength comb, spectrum sum_k cmf(l_k) s_k / 4
+    and power 1 (no dielectric, so the comb never collapses)."""
     bm = scene_empty_beam
     corner, bu, bv, bd = _v(bm.corner[:]), _v(bm.u[:]), _v(bm.v[:]), _v(bm.dir[:])
     cam_o = _basis(cam)[0]
@@ -111,9 +118,16 @@ def _point_film(scene_empty_beam, cam, w, h, n, seed, vp, power_of):
         r = _Rng(i, seed)
         a, b = r.f32(), r.f32()
         o = corner + bu * a + bv * b
-        lam = F(380.0) + F(350.0) * r.f32()
-        cmf = _sample(CMF, lam)
-        power = power_of(lam)
+        u = r.f32()
+        lam = F(380.0) + F(350.0) * u
+        if hero:
+            cmf = np.zeros(3, np.float32)
+            for lk in _comb(u):
+                cmf = cmf + _sample(CMF, lk) * power_of(lk) * F(0.25)
+            power = F(1.0)
+        else:
+            cmf = _sample(CMF, lam)
+            power = power_of(lam)
         seg = F(vp.max_dist)
         for _ in range(4):
             p = o + bd * (r.f32() * seg)
@@ -300,3 +314,66 @@ def test_surface_rim_against_an_analytic_sphere(oracle):
             assert abs(depth[py, px] - tt) < 1e-4 * tt
             checked += 1
     assert checked > 0.2 * w * h
+
+
+# ---- the 4-wavelength hero comb for photons (an extension) ---------------------
+def test_hero_comb_point_deposits_brute_force(oracle):
+    # the comb's spectrum on every deposit of 8 photons through an empty scene
+    sc = Scene(background=0.0)
+    bm = beam((-1.2, -0.3, -0.05), (0.0, 0.0, 0.1), (0.0, 0.1, 0.0), (1.0, 0.0, 0.0))
+    cam = Camera.look_at((0.3, 0.5, 4.0), (0.0, 0.0, 0.0), (0, 1, 0), 40.0, 16.0 / 9.0)
+    vp = A.VolParams(0.3, 0.2, 0.4, 2.5)
+    w, h, n, seed = 96, 54, 8, 77
+    scale = F(1.0 / (np.float64(D65).mean()))
+    for ill, power_of in ((A.ILLUM_E, lambda lam: F(1.0)),
+                          (A.ILLUM_D65, lambda lam: _sample(D65[:, None], lam)[0] * scale)):
+        want, hits = _point_film(bm, cam, w, h, n, seed, vp, power_of, hero=True)
+        got, st = oracle.volume(sc, cam, bm, w, h, n, 0, vp, None, seed, A.EST_POINT, opts(ill, hero_lanes=4),
+                                threads=1)
+        assert st.splats == len(hits) >= 16
+        np.testing.assert_allclose(got, want, rtol=2e-6, atol=0)
+
+
+def test_hero_comb_energy_closed_form_per_channel(oracle):
+    # E[sum_k cmf(l_k) / 4] over the stratified comb = the band mean of each
+    # matching function, so every channel of an isotropic, unextinguished
+    # segment totals n sigma_s <cmf> / (4 pi) * integral dl / d^2
+    sc = Scene(background=0.0)
+    bm = beam((-1.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (1.0, 0.0, 0.0))
+    cam = Camera.look_at((0.0, 0.0, 5.0), (0.0, 0.0, 0.0), (0, 1, 0), 40.0, 16.0 / 9.0)
+    vp = A.VolParams(0.25, 0.0, 0.0, 2.0)
+    n = 1 << 18
+    img, _ = oracle.volume(sc, cam, bm, 160, 90, n, 0, vp, None, 3, A.EST_POINT, opts(A.ILLUM_E, hero_lanes=4),
+                           threads=8)
+    integral = (1 / 5.0) * (math.atan(1 / 5.0) + math.atan(1 / 5.0))
+    c = CMF.astype(np.float64)
+    mean = 5.0 * (c.sum(0) - 0.5 * (c[0] + c[-1])) / 350.0
+    got = img.reshape(-1, 3).astype(np.float64).sum(0)
+    want = n * 0.25 * mean / (4 * math.pi) * integral
+    print("hero comb closed form", want, "oracle", got)
+    assert (np.abs(got / want - 1) < 3e-3).all()
+
+
+def test_hero_comb_collapses_at_the_first_dielectric(oracle):
+    # seen past the prism only (the incoming beam out of frame: its first
+    # segment deposits nothing), the comb has collapsed onto the hero lane,
+    # which is the one-wavelength photon: the films agree to rounding
+    sc, bm = scenes.prism_scene(), scenes.prism_beam()
+    cam = Camera.look_at((1.9, 0.9, 3.0), (1.9, 0.9, 0.0), (0, 1, 0), 40.0, 1.0)
+    vp = A.VolParams(**scenes.HAZE)
+    first, _ = oracle.volume(sc, cam, bm, 96, 96, 1 << 14, 0, vp, None, 4, A.EST_POINT,
+                             opts(A.ILLUM_D65, max_depth=1), threads=8)
+    assert not first.any()
+    one, _ = oracle.volume(sc, cam, bm, 96, 96, 1 << 14, 0, vp, None, 4, A.EST_POINT, opts(A.ILLUM_D65), threads=8)
+    comb, _ = oracle.volume(sc, cam, bm, 96, 96, 1 << 14, 0, vp, None, 4, A.EST_POINT,
+                            opts(A.ILLUM_D65, hero_lanes=4), threads=8)
+    assert one[..., 1].sum() > 0
+    err = np.abs(comb.astype(np.float64) - one).sum() / np.abs(one.astype(np.float64)).sum()
+    print("hero comb past the prism vs one wavelength: rel L1", err)
+    assert err < 1e-5
+    # and in front of the prism the comb differs (four wavelengths per photon)
+    cam2 = scenes.prism_camera(96, 54)
+    a, _ = oracle.volume(sc, cam2, bm, 96, 54, 1 << 14, 0, vp, None, 4, A.EST_POINT, opts(A.ILLUM_D65), threads=8)
+    b, _ = oracle.volume(sc, cam2, bm, 96, 54, 1 << 14, 0, vp, None, 4, A.EST_POINT,
+                         opts(A.ILLUM_D65, hero_lanes=4), threads=8)
+    assert np.abs(a - b).sum() > 1e-3 * np.abs(a).sum()
