@@ -243,6 +243,31 @@ def config_work(O):
     return {k: {kk: round(vv, 4) for kk, vv in v.items()} for k, v in out.items()}
 
 
+def beam_roofline(out, a_peak, chunk=1 << 19):
+    """The product image's beam-estimator deposit kernel: its VALU lane
+    operations and global atomics per dispatch of `chunk` photons (rocprofv3
+    counters of that dispatch size) over the volume pass's time per dispatch
+    measured here (the pass is 98.5 % deposit kernel)."""
+    ms = out.get("prism_on_plane_800x600_volume_ms")
+    n_v = out.get("prism_on_plane_800x600_volume_photons", 0)
+    prof = "prism_on_plane_beam_deposit_counters.md"
+    insts = profile_figure(prof, "valu_insts[k_beam_deposit]")
+    lanes = profile_figure(prof, "lanes_active[k_beam_deposit]")
+    atomics = profile_figure(prof, "global_atomics_per_launch[k_beam_deposit]")
+    if not ms or insts is None or lanes is None:
+        return None
+    sec = ms / 1e3 / -(-n_v // chunk)
+    row = {"kernel": "k_beam_deposit", "bound": "valu", "unit": "T lane-ops/s",
+           "achieved": round(insts * lanes / sec / 1e12, 3), "peak": round(VALU_LANE_PEAK / 1e12, 2)}
+    row["frac"] = round(row["achieved"] / row["peak"], 4)
+    row["valu_busy"] = profile_figure(prof, "valu_busy[k_beam_deposit]")
+    row["lanes_active"] = lanes
+    if atomics is not None and a_peak:
+        row["atomics_gops"] = round(atomics / sec / 1e9, 2)
+        row["atomic_frac"] = round(row["atomics_gops"] / a_peak, 4)
+    return row
+
+
 def config_roofline(out, work, a_peak):
     """Per-configuration model fractions: the wavefront state model's HBM
     bytes over 8 TB/s for every form, and for the plain photon kernels --
@@ -381,6 +406,23 @@ def per_config(ctx, timer, quick):
     out["prism_on_plane_800x600_ms"] = round(wall * 1e3, 1)
     out["prism_on_plane_800x600_mphotons_s"] = round((n_c + n_v) / wall / 1e6, 1)
     out["prism_on_plane_800x600_exposure"] = round(img.exposure, 4)
+    # its volume pass alone (the beam estimator, z-buffered by the surface
+    # pass), timed with HIP events: the deposit kernel's share of the product
+    sc = scenes.wedge_scene().upload(ctx)
+    zb = S.DeviceBuffer(ctx, 800 * 600, None)
+    zb.upload(img.depth)
+    vfilm = S.Film(ctx, 800, 600)
+    hz = scenes.PLANE_HAZE
+    f = lambda: S.render_volumetric_scene(ctx, sc, scenes.plane_camera(800, 600), scenes.plane_beam(), 800, 600, n_v,
+                                          hz["sigma_s"], hz["sigma_t"], hz["g"], hz["max_dist"], zb, 0xBEE5,
+                                          film=vfilm, engine=A.ENGINE_WAVEFRONT, stream=st)
+    ms, _ = timer.run(f, 1)
+    out["prism_on_plane_800x600_volume_ms"] = round(ms, 1)
+    out["prism_on_plane_800x600_volume_photons"] = n_v
+    out["prism_on_plane_800x600_volume_mphotons_s"] = round(n_v / ms / 1e3, 1)
+    vfilm.free()
+    zb.free()
+    sc.close()
     return out
 
 
@@ -624,6 +666,9 @@ def main() -> int:
         result["configs"] = per_config(ctx, timer, args.quick)
         if "configs" in wm:
             result["configs_roofline"] = config_roofline(result["configs"], wm["configs"], a_peak)
+        beam = beam_roofline(result["configs"], a_peak)
+        if beam is not None:
+            result.setdefault("configs_roofline", {})["prism_on_plane_800x600_beam_deposit"] = beam
         log(json.dumps(result["configs"]))
     if not args.no_serve:
         from spectral_amd import serve

@@ -25,7 +25,11 @@ from _metrics import rel_l1_per_channel
 pytestmark = pytest.mark.gpu
 
 FILM_TOL = 5e-3
-ALONE_TOL = 1e-4
+# batched vs one launch per job: float summation order only; the table's
+# largest photon jobs (2^24 photons) put ~10^5 fp32 adds on a hot pixel, in
+# a different order (LDS hash and binned records against direct atomics),
+# and differ by up to 7e-5
+ALONE_TOL = 2e-4
 THREADS = 16
 
 
