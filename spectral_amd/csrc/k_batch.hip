@@ -197,12 +197,12 @@ __global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__res
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
         } else if constexpr (SPLAT == 2) {
             const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
-            BinSink b{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
+            BinSink b{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
                       wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
         } else if constexpr (SPLAT == 3) {
             const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
-            HashBinSink<kSlots> hb{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
+            HashBinSink<kSlots> hb{hash, BinSink{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap,
                                                       pool, P.key_base, P.film, wstate, fw,
                                                       tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hb);

@@ -119,10 +119,12 @@ __device__ inline uint32_t mag_bits(const BinRec &r) {
     return max(a, max(b, c));
 }
 
+// Count pass: streams the keys only.  Each group's run of a strip is
+// reserved padded to kRunAlign slots, so every run starts on a multiple of
+// kRunAlign within its strip.
 __global__ __launch_bounds__(kBinThreads) void k_bins_count(BinState st) {
     __shared__ uint32_t keys[kGroupSlots];
     __shared__ uint32_t cnt[kGroupSlots];
-    __shared__ uint32_t mx[kGroupSlots];
     __shared__ uint32_t s_n[kChunksPerBlock];
     __shared__ uint32_t s_m;
     const uint32_t groups = kBinPools * (st.pool_cap / kBinChunk) / kChunksPerBlock;
@@ -131,36 +133,24 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_count(BinState st) {
         for (uint32_t s = threadIdx.x; s < kGroupSlots; s += blockDim.x) {
             keys[s] = kHashEmpty;
             cnt[s] = 0;
-            mx[s] = 0;
         }
         if (threadIdx.x == 0) s_m = 0;
         const uint32_t c0 = g * kChunksPerBlock;
         load_group_counts(st, c0, s_n);
         __syncthreads();
-        // All records first (16 loads in flight per thread), then the LDS work.
-        uint32_t strip[kRecsPerThread], mag[kRecsPerThread];
+        // All keys first (16 loads in flight per thread), then the LDS work.
+        uint32_t strip[kRecsPerThread];
 #pragma unroll
         for (uint32_t k = 0; k < kRecsPerThread; ++k) {
             uint32_t at;
-            if (group_rec(s_n, c0, k, at)) {
-                BinRec r = st.recs[at];
-                strip[k] = r.key >> kStripLog2;
-                mag[k] = mag_bits(r);
-            } else {
-                strip[k] = kHashEmpty;
-                mag[k] = 0;
-            }
+            strip[k] = group_rec(s_n, c0, k, at) ? st.rkey[at] >> kStripLog2 : kHashEmpty;
         }
 #pragma unroll
         for (uint32_t k = 0; k < kRecsPerThread; ++k) {
             const bool act = strip[k] != kHashEmpty;
             uint32_t sl = act ? strip_slot(keys, strip[k]) : kHashEmpty;
             slot_add(cnt, sl, act && sl != kHashEmpty);
-            if (act && sl != kHashEmpty) atomicMax(mx + sl, mag[k]);
-            if (act && sl == kHashEmpty) {
-                atomicAdd(st.ovf_count + strip[k], 1u);
-                atomicMax(st.ovf_max + strip[k], mag[k]);
-            }
+            if (act && sl == kHashEmpty) atomicAdd(st.ovf_count + strip[k], 1u);
         }
         __syncthreads();
         const uint32_t sub = g % kSub;
@@ -168,8 +158,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_count(BinState st) {
             const uint32_t key = keys[s];
             if (key == kHashEmpty) continue;
             const size_t i = (size_t)key * kSub + sub;
-            uint32_t rel = atomicAdd(st.sub_count + i, cnt[s]);
-            atomicMax(st.sub_max + i, mx[s]);
+            uint32_t rel = atomicAdd(st.sub_count + i, run_pad(cnt[s]));
             uint32_t j = atomicAdd(&s_m, 1u);
             st.glist[(size_t)g * kGroupSlots + j] = make_uint2(key, rel);
         }
@@ -178,23 +167,21 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_count(BinState st) {
     }
 }
 
-// Per strip: total records, largest magnitude, and the sub-bucket counts
-// turned into offsets within the strip (sub-buckets 0..kSub-1, then the
-// overflow bucket).
+// Per strip: the slots it uses (padded runs, then its overflow records)
+// and the sub-bucket sizes turned into offsets within the strip
+// (sub-buckets 0..kSub-1, then the overflow bucket).
 __global__ __launch_bounds__(256) void k_bins_totals(BinState st) {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= st.n_strips) return;
-    uint32_t c[kSub], m = st.ovf_max[s];
+    uint32_t c[kSub];
     const uint4 *sc = (const uint4 *)(st.sub_count + (size_t)s * kSub);
-    const uint4 *sm = (const uint4 *)(st.sub_max + (size_t)s * kSub);
 #pragma unroll
     for (uint32_t q = 0; q < kSub / 4; ++q) {
-        uint4 v = sc[q], w = sm[q];
+        uint4 v = sc[q];
         c[4 * q] = v.x;
         c[4 * q + 1] = v.y;
         c[4 * q + 2] = v.z;
         c[4 * q + 3] = v.w;
-        m = max(m, max(max(w.x, w.y), max(w.z, w.w)));
     }
     uint32_t run = 0;
     uint4 *so = (uint4 *)(st.sub_count + (size_t)s * kSub);
@@ -214,10 +201,10 @@ __global__ __launch_bounds__(256) void k_bins_totals(BinState st) {
     const uint32_t ovf = st.ovf_count[s];
     st.ovf_count[s] = run;
     st.strip_total[s] = run + ovf;
-    st.strip_max[s] = m;
 }
 
-// Exclusive scans of the strip record counts and of the accumulation parts.
+// Exclusive scans of the strips' slots (each padded to kRunAlign, so every
+// strip starts aligned) and of the accumulation parts.
 __global__ __launch_bounds__(1024) void k_bins_scan(BinState st) {
     __shared__ uint32_t sa[1024], sp[1024];
     const uint32_t n = st.n_strips, T = blockDim.x;
@@ -226,7 +213,7 @@ __global__ __launch_bounds__(1024) void k_bins_scan(BinState st) {
     uint32_t a = 0, p = 0;
     for (uint32_t i = b0; i < b1; ++i) {
         uint32_t c = st.strip_total[i];
-        a += c;
+        a += run_pad(c);
         p += (c + kPartRecs - 1) / kPartRecs;
     }
     sa[threadIdx.x] = a;
@@ -246,7 +233,7 @@ __global__ __launch_bounds__(1024) void k_bins_scan(BinState st) {
         uint32_t c = st.strip_total[i];
         st.strip_off[i] = ra;
         st.part_off[i] = rp;
-        ra += c;
+        ra += run_pad(c);
         rp += (c + kPartRecs - 1) / kPartRecs;
     }
     if (threadIdx.x == T - 1) {
@@ -287,10 +274,14 @@ __global__ __launch_bounds__(256) void k_bins_macc_zero(BinState st) {
         st.macc[i] = 0ull;
 }
 
+// Scatter pass: each record to its slot in strip order, the slack of every
+// padded run filled with pad records, and each strip's largest magnitude
+// (for the fixed-point scale) taken on the way.
 __global__ __launch_bounds__(kBinThreads) void k_bins_scatter(BinState st) {
     __shared__ uint32_t keys[kGroupSlots];
     __shared__ uint32_t cnt[kGroupSlots];
     __shared__ uint32_t base[kGroupSlots];
+    __shared__ uint32_t mx[kGroupSlots];
     __shared__ uint32_t s_n[kChunksPerBlock];
     const uint32_t groups = kBinPools * (st.pool_cap / kBinChunk) / kChunksPerBlock;
     for (uint32_t g = blockIdx.x; g < groups; g += gridDim.x) {
@@ -298,6 +289,7 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_scatter(BinState st) {
         for (uint32_t s = threadIdx.x; s < kGroupSlots; s += blockDim.x) {
             keys[s] = kHashEmpty;
             cnt[s] = 0;
+            mx[s] = 0;
         }
         const uint32_t c0 = g * kChunksPerBlock;
         load_group_counts(st, c0, s_n);
@@ -309,7 +301,10 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_scatter(BinState st) {
         for (uint32_t k = 0; k < kRecsPerThread; ++k) {
             uint32_t at;
             if (group_rec(s_n, c0, k, at)) {
-                r[k] = st.recs[at];
+                r[k].key = st.rkey[at];
+                r[k].x = st.rval[3 * (size_t)at + 0];
+                r[k].y = st.rval[3 * (size_t)at + 1];
+                r[k].z = st.rval[3 * (size_t)at + 2];
                 live |= 1u << k;
             } else {
                 r[k].key = 0;
@@ -329,11 +324,28 @@ __global__ __launch_bounds__(kBinThreads) void k_bins_scatter(BinState st) {
             const uint32_t sl = act ? slot_find(keys, strip) : kHashEmpty;
             const uint32_t rank = slot_add(cnt, sl, act && sl != kHashEmpty);
             if (act) {
-                const uint32_t pos = sl != kHashEmpty ? base[sl] + rank
-                                                      : st.strip_off[strip] + st.ovf_count[strip] +
-                                                            atomicAdd(st.ovf_cur + strip, 1u);
+                uint32_t pos;
+                if (sl != kHashEmpty) {
+                    pos = base[sl] + rank;
+                    atomicMax(mx + sl, mag_bits(r[k]));
+                } else {
+                    pos = st.strip_off[strip] + st.ovf_count[strip] + atomicAdd(st.ovf_cur + strip, 1u);
+                    atomicMax(st.strip_max + strip, mag_bits(r[k]));
+                }
                 st.sorted[pos] = r[k];
             }
+        }
+        __syncthreads();
+        // Pads after each run, and the runs' largest magnitudes.
+        for (uint32_t j = threadIdx.x; j < m; j += blockDim.x) {
+            const uint32_t strip = st.glist[(size_t)g * kGroupSlots + j].x;
+            const uint32_t sl = slot_find(keys, strip);
+            const uint32_t n = cnt[sl];
+            BinRec pad;
+            pad.key = kPadKey;
+            pad.x = pad.y = pad.z = 0.0f;
+            for (uint32_t q = n; q < run_pad(n); ++q) st.sorted[base[sl] + q] = pad;
+            atomicMax(st.strip_max + strip, mx[sl]);
         }
         __syncthreads();
     }
@@ -380,7 +392,7 @@ __global__ __launch_bounds__(256) void k_bins_accumulate(BinState st, const uint
         const uint32_t nparts = st.part_off[s + 1] - st.part_off[s];
         const uint32_t r0 = st.strip_off[s] + q * kPartRecs;
         uint32_t r1 = r0 + kPartRecs;
-        if (r1 > st.strip_off[s + 1]) r1 = st.strip_off[s + 1];
+        if (r1 > st.strip_off[s] + st.strip_total[s]) r1 = st.strip_off[s] + st.strip_total[s];
         const uint32_t mb = st.strip_max[s];
         // Non-finite deposits keep fp32 sums, so inf / NaN reach the film.
         const uint32_t macc_i = nparts > 1 ? st.strip_macc[s] : kNoMacc;
@@ -397,7 +409,7 @@ __global__ __launch_bounds__(256) void k_bins_accumulate(BinState st, const uint
 #pragma unroll
             for (uint32_t u = 0; u < U; ++u) {
                 uint32_t i = i0 + u * blockDim.x + threadIdx.x;
-                if (i >= r1) continue;
+                if (i >= r1 || r[u].key == kPadKey) continue;
                 uint32_t l = r[u].key & (kStripPix - 1);
                 if (fixed) {
                     atomicAdd(acc + 3 * l + 0, to_fixed(r[u].x, e));

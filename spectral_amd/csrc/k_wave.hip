@@ -373,7 +373,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
         if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
         __syncthreads();
         const uint32_t fw = MODE == 3 ? P.scr_w : P.w;
-        HashBinSink<kSplatSlots> sink{hash, BinSink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap,
+        HashBinSink<kSplatSlots> sink{hash, BinSink{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap,
                                                     xcc_id() % kBinPools, P.key_base, P.film, L.wstate, fw,
                                                     tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
         step(sink);
@@ -383,7 +383,7 @@ __device__ inline void ph_body(const SceneData &S, const float4 *tbl, const Phot
         if (threadIdx.x < 8) L.wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
         __syncthreads();
         const uint32_t fw = MODE == 3 ? P.scr_w : P.w;
-        BinSink sink{bins.recs, bins.pool_next, bins.chunk_fill, bins.pool_cap, xcc_id() % kBinPools, P.key_base,
+        BinSink sink{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap, xcc_id() % kBinPools, P.key_base,
                      P.film, L.wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
         step(sink);
         bin_wave_finish(L.wstate, bins.chunk_fill);

@@ -131,16 +131,16 @@ BinLayout bins_layout(const BinJobMap &map, size_t want, uint32_t bin_log2) {
     L.nj = nj;
     size_t off = 0;
     auto take = [&](size_t bytes) { size_t o = off; off += a256(bytes); return o; };
-    L.o_recs = take(cap * sizeof(BinRec));
-    L.o_sorted = take(cap * sizeof(BinRec));
+    L.o_recs = take(cap * 4);       // keys
+    L.o_vals = take(cap * 12);      // values
+    L.o_sorted = take(sorted_capacity(cap, n) * sizeof(BinRec));
     const size_t groups = cap / kGroupRecs;
-    L.zero_words = n * (2 * kSub + 3);  // sub_count | sub_max | ovf_count | ovf_max | ovf_cur
+    L.zero_words = n * (kSub + 3);  // sub_count | ovf_count | ovf_cur | strip_max
     L.o_pool = take(kBinPools * 4);
     L.o_partn = take(4);
     L.o_fill = take(chunks * 4);
     L.o_zero = take(L.zero_words * 4);
     L.o_tot = take(n * 4);
-    L.o_max = take(n * 4);
     L.o_off = take((n + 1) * 4);
     L.o_poff = take((n + 1) * 4);
     L.o_glist = take(groups * kGroupSlots * sizeof(uint2));
@@ -165,19 +165,18 @@ BinLayout bins_layout(const BinJobMap &map, size_t want, uint32_t bin_log2) {
 
 void bins_bind(char *b, const BinLayout &L, const BinJobMap &map, BinState &st, const uint32_t **job_shared) {
     const size_t n = L.n;
-    st.recs = (BinRec *)(b + L.o_recs);
+    st.rkey = (uint32_t *)(b + L.o_recs);
+    st.rval = (float *)(b + L.o_vals);
     st.sorted = (BinRec *)(b + L.o_sorted);
     st.pool_next = (uint32_t *)(b + L.o_pool);
     st.chunk_fill = (uint32_t *)(b + L.o_fill);
     uint32_t *z = (uint32_t *)(b + L.o_zero);
     st.sub_count = z;
-    st.sub_max = z + n * kSub;
-    st.ovf_count = z + 2 * n * kSub;
-    st.ovf_max = st.ovf_count + n;
-    st.ovf_cur = st.ovf_max + n;
+    st.ovf_count = z + n * kSub;
+    st.ovf_cur = st.ovf_count + n;
+    st.strip_max = st.ovf_cur + n;
     st.zero_words = L.zero_words;
     st.strip_total = (uint32_t *)(b + L.o_tot);
-    st.strip_max = (uint32_t *)(b + L.o_max);
     st.strip_off = (uint32_t *)(b + L.o_off);
     st.part_off = (uint32_t *)(b + L.o_poff);
     st.glist = (uint2 *)(b + L.o_glist);

@@ -708,8 +708,8 @@ int spx::batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_
 
 double spx::job_queue_bytes(const spx_ctx *ctx, const spx_job &job) {
     if (job.kind == SPX_JOB_CAMERA) return 0.0;
-    if (ctx->tuning.splat_mode >= 2)  // records and their sorted copy, 16 B each
-        return 32.0 * (double)job.n_photons * records_per_photon(job.kind == SPX_JOB_LIGHTTRACE);
+    if (ctx->tuning.splat_mode >= 2)  // records and their padded sorted copy
+        return kRoundBytesPerRecord * (double)job.n_photons * records_per_photon(job.kind == SPX_JOB_LIGHTTRACE);
     size_t n = (size_t)3 * photon_film_w(job) * photon_film_h(job);
     uint32_t R = replica_count(ctx, n);
     return R > 1 ? (double)R * n * sizeof(float) : 0.0;

@@ -1,23 +1,26 @@
 // spx_bins.hpp -- binned photon deposits: records instead of film atomics.
 //
-// Deposits are appended as 16-byte records {key, X, Y, Z} (key = the job's
-// key base + tile-ordered pixel) into chunked pools, one pool per XCD (a
-// wave whose pool is full takes chunks from the others).  Each wave owns
-// one 512-record chunk at a time, so an append costs one LDS read-modify-
-// write by the wave's leader and one coalesced 16-byte store per lane; a new
-// chunk costs one global atomic per 512 records.  After the trace, the
-// records are counted per strip (a 1024-pixel film tile, 2^tw x 2^(10-tw)
-// with tw chosen per call -- 32x32 by default: spatially compact, so the
-// few tiles a beam or caustic crosses collect its deposits),
-// scanned, scattered into strip order and accumulated strip by strip in LDS
-// as 64-bit fixed point scaled to the strip's largest deposit and record
-// count (integer LDS atomics: order-independent, resolution finer than the
-// fp32 rounding of a single add).  A strip's film pixels are written with
-// plain stores by their one owner; strips split over several blocks first
-// sum the blocks' fixed-point tiles with 64-bit integer atomics.  Films are
-// therefore bit-reproducible unless two jobs share one (fp32 atomics between
-// jobs) or a pool fills: a full pool falls back to direct film atomics, so
-// results never depend on capacity.
+// Deposits are appended as records {key, X, Y, Z} (key = the job's key base
+// + tile-ordered pixel) into chunked pools, one pool per XCD (a wave whose
+// pool is full takes chunks from the others), keys and values in separate
+// arrays (4 + 12 bytes), so the count pass streams only the keys.  Each
+// wave owns one 512-record chunk at a time, so an append costs one LDS
+// read-modify-write by the wave's leader and two coalesced stores per lane;
+// a new chunk costs one global atomic per 512 records.  After the trace,
+// the records are counted per strip (a 1024-pixel film tile, 2^tw x
+// 2^(10-tw) with tw chosen per call -- 32x32 by default: spatially
+// compact, so the few tiles a beam or caustic crosses collect its
+// deposits), scanned, scattered into strip order -- each group's run of a
+// strip padded to kRunAlign records, so that runs written by different
+// blocks never share a memory segment -- and accumulated strip by strip in
+// LDS as 64-bit fixed point scaled to the strip's largest deposit and
+// record count (integer LDS atomics: order-independent, resolution finer
+// than the fp32 rounding of a single add).  A strip's film pixels are
+// written with plain stores by their one owner; strips split over several
+// blocks first sum the blocks' fixed-point tiles with 64-bit integer
+// atomics.  Films are therefore bit-reproducible unless two jobs share one
+// (fp32 atomics between jobs) or a pool fills: a full pool falls back to
+// direct film atomics, so results never depend on capacity.
 #pragma once
 
 #include "spx_device.hpp"
@@ -37,6 +40,13 @@ constexpr uint32_t kSub = 16;            // sub-counters per strip
 constexpr uint32_t kGroupSlots = 1024;   // distinct strips per group tracked in LDS
 constexpr uint32_t kGroupRecs = kChunksPerBlock * kBinChunk;  // records per group
 constexpr uint32_t kNoMacc = 0xFFFFFFFFu;
+// Sorted runs start on multiples of kRunAlign records (32 bytes, the memory
+// write granule: unaligned runs cost read-modify-write traffic at their
+// shared edges; 2, 4 and 8 records measured 29.5, 30.5 and 32.7 GB per
+// 5e8-record round); the slack after a run holds pad records (key kPadKey),
+// which the accumulation skips.
+constexpr uint32_t kRunAlign = 2;
+constexpr uint32_t kPadKey = 0xFFFFFFFFu;
 constexpr uint32_t kChunkEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kChunkFull = 0xFFFFFFFEu;
 
@@ -45,6 +55,19 @@ struct __align__(16) BinRec {
     float x, y, z;
 };
 
+__host__ __device__ constexpr uint32_t run_pad(uint32_t n) { return (n + kRunAlign - 1) & ~(kRunAlign - 1); }
+
+// Sorted-array slots for `cap` records over `n_strips` strips: every group
+// run and every strip can end with up to kRunAlign - 1 pads, and a group's
+// 4096 records fall in at most kGroupSlots runs (cap / 4 over all groups).
+__host__ __device__ constexpr size_t sorted_capacity(size_t cap, size_t n_strips) {
+    return cap + (size_t)(kRunAlign - 1) * (cap / (kGroupRecs / kGroupSlots) + n_strips) + kRunAlign;
+}
+// Device bytes per record of a deposit round: key, value and the worst
+// case of the padded sorted copy.
+constexpr double kRoundBytesPerRecord =
+    16.0 + 16.0 * (1.0 + (double)(kRunAlign - 1) * kGroupSlots / kGroupRecs);
+
 // Device-side state of one deposit round (all pointers device memory).
 // Records are handled in groups of kChunksPerBlock chunks.  The count pass
 // tallies each group's strips in LDS and reserves the group's share of every
@@ -52,20 +75,19 @@ struct __align__(16) BinRec {
 // spread so concurrent groups rarely meet on one address), leaving a list of
 // {strip, offset in sub-bucket} per group for the scatter pass.
 struct BinState {
-    BinRec *recs;          // kBinPools * pool_cap records
-    BinRec *sorted;        // same size
+    uint32_t *rkey;        // kBinPools * pool_cap record keys
+    float *rval;           // 3 floats per record
+    BinRec *sorted;        // sorted_capacity(pool records, n_strips) slots
     uint32_t *pool_next;   // [kBinPools] records taken per pool
     uint32_t *chunk_fill;  // [kBinPools * pool_cap / kBinChunk]
-    // Zeroed each round, contiguous: sub_count | sub_max | ovf_count | ovf_max | ovf_cur.
-    uint32_t *sub_count;   // [n_strips * kSub] counts; after the totals pass, offsets within the strip
-    uint32_t *sub_max;     // [n_strips * kSub] largest |x|,|y|,|z| bit pattern
+    // Zeroed each round, contiguous: sub_count | ovf_count | ovf_cur | strip_max.
+    uint32_t *sub_count;   // [n_strips * kSub] padded run lengths; after the totals pass, offsets in the strip
     uint32_t *ovf_count;   // [n_strips] records of groups whose LDS table was full; then their offset
-    uint32_t *ovf_max;     // [n_strips]
     uint32_t *ovf_cur;     // [n_strips] scatter cursor of those records
-    size_t zero_words;     // words from sub_count to the end of ovf_cur
-    uint32_t *strip_total; // [n_strips]
-    uint32_t *strip_max;   // [n_strips]
-    uint32_t *strip_off;   // [n_strips + 1]
+    uint32_t *strip_max;   // [n_strips] largest |x|,|y|,|z| bit pattern (the scatter pass)
+    size_t zero_words;     // words from sub_count to the end of strip_max
+    uint32_t *strip_total; // [n_strips] slots the strip uses (padded runs, then its overflow records)
+    uint32_t *strip_off;   // [n_strips + 1] first slot (strips start on kRunAlign multiples)
     uint32_t *part_off;    // [n_strips + 1]
     uint32_t *part_next;   // [1]
     uint2 *glist;          // [groups * kGroupSlots] {strip, offset} per group
@@ -140,7 +162,8 @@ __device__ __noinline__ uint32_t grab_chunk(uint32_t *pool_next, uint32_t pool_c
 // A film sink that appends records.  `wstate` is LDS: {chunk, fill} per wave
 // of the block, initialised to kChunkEmpty by the kernel.
 struct BinSink {
-    BinRec *recs;
+    uint32_t *rkey;
+    float *rval;
     uint32_t *pool_next;
     uint32_t *chunk_fill;
     uint32_t pool_cap;
@@ -190,12 +213,11 @@ struct BinSink {
             film_add(film, idx, v);
             return;
         }
-        BinRec r;
-        r.key = key_base + key;
-        r.x = v.x;
-        r.y = v.y;
-        r.z = v.z;
-        recs[base + pre] = r;
+        const uint32_t at = base + pre;
+        rkey[at] = key_base + key;
+        rval[3 * (size_t)at + 0] = v.x;
+        rval[3 * (size_t)at + 1] = v.y;
+        rval[3 * (size_t)at + 2] = v.z;
     }
 };
 

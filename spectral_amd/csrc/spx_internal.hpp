@@ -186,7 +186,7 @@ struct BinJobMap {
 // counters, group lists, split-strip accumulators, then the job map.
 struct BinLayout {
     size_t cap = 0, n = 0, nj = 0, zero_words = 0, max_multi = 0;
-    size_t o_recs = 0, o_sorted = 0, o_pool = 0, o_partn = 0, o_fill = 0, o_zero = 0, o_tot = 0, o_max = 0;
+    size_t o_recs = 0, o_vals = 0, o_sorted = 0, o_pool = 0, o_partn = 0, o_fill = 0, o_zero = 0, o_tot = 0;
     size_t o_off = 0, o_poff = 0, o_glist = 0, o_gcount = 0, o_macc = 0, o_smacc = 0, o_maccs = 0, o_maccn = 0;
     size_t o_map = 0, m_key = 0, m_npix = 0, m_w = 0, m_sh = 0, m_film = 0, m_end = 0;
     size_t total = 0;
