@@ -92,6 +92,105 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
     }
 }
 
+// The same photon item with every lane of the wave in the loop until the
+// wave's last photon ends, so that each of a bounce's deposit candidates
+// reaches the sink with the wave converged: the transport and the deposit
+// arithmetic stay per lane (the same operations, and random draws in the
+// same order, as the bounce functions above), the sink's bookkeeping is
+// wave-uniform.  Each wave takes an equal share of the item's photons and
+// hands a lane whose photon has ended the share's next one from a
+// wave-uniform counter, so the wave's lanes run out of photons within one
+// photon's life of each other (a fixed stride of photons per lane leaves
+// the wave waiting on its longest lane).  A photon's result depends only on
+// its index, not on the lane that traces it.
+template <class WS>
+__device__ inline void conv_photon_item(const SceneData &S, const float4 *tbl, const PhotonParams &P,
+                                        uint32_t kind, uint32_t start, uint32_t count, WS &sink) {
+    const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = __lane_id();
+    const uint32_t per = (count + waves - 1) / waves;
+    const uint32_t w0 = min(wave * per, count), w1 = min(w0 + per, count);
+    uint32_t next = w0 + 64;  // the share's next photon not yet handed out (wave-uniform)
+    uint32_t f = w0 + lane;
+    bool live = f < w1;
+    bool spread = false;  // the photon has scattered diffusely (its deposits skip the LDS table)
+    uint32_t b = 0;
+    PhotonState st;
+    if (live) photon_begin(P, tbl, P.photon_base + start + f, st);
+    const bool screen = kind == SPX_JOB_LIGHTTRACE;
+    const bool legacy = uniform_u32(P.estimator) == SPX_EST_LEGACY;
+    const bool iso = uniform_f32(P.vp.g) == 0.0f;
+    const f3 cam_o = ld3(P.proj.o);
+    const float *zbuf = uniform_ptr(P.zbuf);
+    while (__ballot(live)) {
+        Hit h;
+        int mi = -1;
+        bool ends = false;  // the photon ends after this bounce's deposits
+        if (screen) {  // lighttrace.rs:133-151
+            bool landed = false;
+            ScreenTexels T;
+            if (live) {
+                mi = scene_hit(S, st.ro, st.rd, h);
+                float ss, rr, stt;
+                const bool on = screen_coords(P, st.ro, st.rd, ss, rr, stt);
+                const float scene_t = mi >= 0 ? h.t : INFINITY;
+                const float screen_t = on ? stt : INFINITY;
+                landed = screen_t < scene_t;
+                if (landed) T = screen_texels(P, ss, rr, st.cmf, st.power);
+                ends = landed || mi < 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Dep d;
+                d.ok = false;
+                if (landed) d = screen_texel(P, T, k);
+                sink.deposit(d, spread);
+            }
+        } else {  // volume.rs:130-150 (point) / 51-64 (legacy)
+            float seg_len = 0.0f;
+            if (live) {
+                mi = scene_hit(S, st.ro, st.rd, h);
+                seg_len = mi >= 0 ? h.t : P.vp.max_dist;
+                ends = mi < 0;
+            }
+            const f3 cmf = st.cmf;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                Dep d;
+                d.ok = false;
+                if (live) {
+                    if (legacy) d = point_sample<true, true>(P, st, cmf, seg_len, st.rng, cam_o, zbuf);
+                    else if (iso) d = point_sample<false, true>(P, st, cmf, seg_len, st.rng, cam_o, zbuf);
+                    else d = point_sample<false, false>(P, st, cmf, seg_len, st.rng, cam_o, zbuf);
+                }
+                sink.deposit(d, spread);
+            }
+        }
+        bool done = false;
+        if (live) {
+            if (!ends) {
+                photon_scatter(S, tbl, P, mi, h, st);
+                if (S.mats[mi].kind == SPX_MAT_LAMBERTIAN) spread = true;
+            }
+            ++b;
+            done = ends || b >= P.max_depth;
+        }
+        const uint64_t m = __ballot(done);
+        if (m) {
+            const uint32_t mine = next + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            next += (uint32_t)__popcll(m);
+            if (done) {
+                f = mine;
+                live = f < w1;
+                if (live) {
+                    photon_begin(P, tbl, P.photon_base + start + f, st);
+                    spread = false;
+                    b = 0;
+                }
+            }
+        }
+    }
+}
+
 // Shared by both kernels: block-level item claim (home XCD queue first, then
 // the others) and scene staging when the item's job changes scene.
 struct BatchBlock {
@@ -162,9 +261,11 @@ __global__ __launch_bounds__(256, 4) void k_batch_camera(const BatchJob *__restr
 // draining to binned records.
 // THREADS: block size; the LDS hash holds 8 slots per thread (2048 at 256
 // threads; 8192 at 1024 threads, one block filling a compute unit, so all of
-// the unit's photons share one 128 KiB table).
+// the unit's photons share one 128 KiB table).  Capped at 128 VGPRs (four
+// waves per SIMD), so a concurrent camera round's waves still fit beside
+// three photon blocks per compute unit.
 template <int SPLAT, int THREADS = 256>
-__global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__restrict__ jobs,
+__global__ __launch_bounds__(THREADS, 4) void k_batch_photons(const BatchJob *__restrict__ jobs,
                                                            const BatchItem *__restrict__ items, BatchQueues *q,
                                                            BinState bins) {
     constexpr int kSlots = THREADS / 256 * kBatchSlots;
@@ -173,15 +274,16 @@ __global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__res
     __shared__ uint32_t sk[SPLAT == 1 || SPLAT == 3 ? kSlots : 1];
     __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kSlots : 1];
     __shared__ uint32_t sfill;
-    __shared__ uint32_t wstate[2 * kWaves];  // binned deposits: {chunk, fill} per wave
-    LdsSplat<kSlots, SPLAT == 3 ? kHashBinProbes : 8> hash{sk, sv, &sfill, nullptr};
+    LdsSplat<kSlots, SPLAT == 3 ? kHashBinProbes : 8, false> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
         B.scene = nullptr;
         B.tbl_src = nullptr;
     }
     if (SPLAT == 1 || SPLAT == 3) hash.init();
-    if (threadIdx.x < 2 * kWaves) wstate[threadIdx.x] = (threadIdx.x & 1) ? 0u : kChunkEmpty;
-    const uint32_t pool = xcc_id() % kBinPools;
+    // binned deposits: the wave-uniform record sink (its chunk state in registers)
+    WaveSink<kSlots, SPLAT == 3, THREADS> ws{{sk, sv, &sfill, nullptr}, bins.rkey, bins.rval, bins.pool_next,
+                                              bins.chunk_fill, bins.pool_cap, xcc_id() % kBinPools,
+                                              bins.tile_wlog2};
     const uint32_t home = blockIdx.x % kXcdQueues;
     uint32_t qi = 0;
     while (true) {
@@ -195,18 +297,10 @@ __global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__res
         if constexpr (SPLAT == 0) {
             GlobalSink g{deposit_film(P)};
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
-        } else if constexpr (SPLAT == 2) {
-            const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
-            BinSink b{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap, pool, P.key_base, P.film,
-                      wstate, fw, tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2};
-            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, b);
-        } else if constexpr (SPLAT == 3) {
-            const uint32_t fw = kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w;
-            HashBinSink<kSlots> hb{hash, BinSink{bins.rkey, bins.rval, bins.pool_next, bins.chunk_fill, bins.pool_cap,
-                                                      pool, P.key_base, P.film, wstate, fw,
-                                                      tiles_across(fw, bins.tile_wlog2), bins.tile_wlog2}};
-            batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hb);
-            hb.flush();
+        } else if constexpr (SPLAT >= 2) {
+            ws.job(P.key_base, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w, P.film);
+            conv_photon_item(B.S, B.tbl, P, kind, item.start, item.count, ws);
+            ws.flush();
         } else {
             hash.film = deposit_film(P);
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, hash);
@@ -214,7 +308,7 @@ __global__ __launch_bounds__(THREADS) void k_batch_photons(const BatchJob *__res
         }
         __syncthreads();
     }
-    if (SPLAT >= 2) bin_wave_finish(wstate, bins.chunk_fill);
+    if (SPLAT >= 2) ws.finish();
 }
 
 // Zeroes the claim counters of every round's queues (the graph replays the

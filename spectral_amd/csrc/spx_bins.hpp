@@ -134,6 +134,14 @@ __host__ __device__ inline uint32_t tile_key(uint32_t idx, uint32_t width, uint3
     return (tile << kStripLog2) | local;
 }
 
+// tile_key of pixel (px, py): no division.
+__host__ __device__ inline uint32_t tile_key_xy(uint32_t px, uint32_t py, uint32_t tiles_x, uint32_t tw) {
+    const uint32_t th = kStripLog2 - tw;
+    const uint32_t tile = (py >> th) * tiles_x + (px >> tw);
+    const uint32_t local = ((py & ((1u << th) - 1u)) << tw) | (px & ((1u << tw) - 1u));
+    return (tile << kStripLog2) | local;
+}
+
 // Film pixel (px, py) of in-tile position k of tile `tile` (tile_key's inverse).
 __host__ __device__ inline void tile_pixel(uint32_t tile, uint32_t k, uint32_t tiles_x, uint32_t tw, uint32_t &px,
                                            uint32_t &py) {
@@ -178,10 +186,7 @@ struct BinSink {
     __device__ inline void add(uint32_t idx, f3 v) { append(tile_key(idx, width, tiles_x, tile_wlog2), idx, v); }
     // The same deposit when its pixel (px, py) is at hand: no division.
     __device__ inline void add_xy(uint32_t px, uint32_t py, uint32_t idx, f3 v) {
-        const uint32_t th = kStripLog2 - tile_wlog2;
-        const uint32_t tile = (py >> th) * tiles_x + (px >> tile_wlog2);
-        const uint32_t local = ((py & ((1u << th) - 1u)) << tile_wlog2) | (px & ((1u << tile_wlog2) - 1u));
-        append((tile << kStripLog2) | local, idx, v);
+        append(tile_key_xy(px, py, tiles_x, tile_wlog2), idx, v);
     }
 
     __device__ inline void append(uint32_t key, uint32_t idx, f3 v) {

@@ -383,6 +383,15 @@ __device__ inline float phase_hg(float g, float cos_theta) {
 // ---------------------------------------------------------------------------
 // Film splat: hardware fp32 atomic add (global_atomic_add_f32, no return).
 // ---------------------------------------------------------------------------
+// One film deposit a segment may make: pixel (px, py) = index idx, value v;
+// ok = false when there is none.
+struct Dep {
+    uint32_t px, py, idx;
+    f3 v;
+    bool ok;
+};
+
+// Adds v to film pixel idx.
 __device__ inline void film_add(float *film, uint32_t idx, f3 v) {
     unsafeAtomicAdd(film + 3 * idx + 0, v.x);
     unsafeAtomicAdd(film + 3 * idx + 1, v.y);

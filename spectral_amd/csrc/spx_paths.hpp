@@ -243,25 +243,54 @@ __device__ inline bool screen_coords(const PhotonParams &P, f3 o, f3 d, float &s
     return false;
 }
 
-// Screen::splat (texel_bilinear + 4-texel deposit), lighttrace.rs:65-91.
-template <class Sink>
-__device__ inline void screen_splat(const PhotonParams &P, float s, float r, f3 c, float power, Sink &sink) {
+// Screen::splat's bilinear footprint (texel_bilinear, lighttrace.rs:65-91):
+// texel k = 0..3 is (x0,y0), (x1,y0), (x0,y1), (x1,y1), weighted
+// (1-dx)(1-dy), dx(1-dy), (1-dx)dy, dx dy, of the colour c * power.
+struct ScreenTexels {
+    uint32_t x0, y0, x1, y1;
+    float w[4];
+    f3 cp;
+};
+
+__device__ inline ScreenTexels screen_texels(const PhotonParams &P, float s, float r, f3 c, float power) {
+    ScreenTexels T;
     float W = (float)P.scr_w, H = (float)P.scr_h;
     float fx = s * W - 0.5f;
     fx = fx < 0.0f ? 0.0f : (fx > W - 1.0f ? W - 1.0f : fx);
     float fy = (1.0f - r) * H - 0.5f;
     fy = fy < 0.0f ? 0.0f : (fy > H - 1.0f ? H - 1.0f : fy);
-    uint32_t x0 = (uint32_t)floorf(fx), y0 = (uint32_t)floorf(fy);
-    uint32_t x1 = x0 + 1 < P.scr_w - 1 ? x0 + 1 : P.scr_w - 1;
-    uint32_t y1 = y0 + 1 < P.scr_h - 1 ? y0 + 1 : P.scr_h - 1;
-    float dx = fx - (float)x0, dy = fy - (float)y0;
-    float w00 = (1.0f - dx) * (1.0f - dy), w10 = dx * (1.0f - dy);
-    float w01 = (1.0f - dx) * dy, w11 = dx * dy;
-    f3 cp = c * power;
-    deposit_xy(sink, x0, y0, y0 * P.scr_w + x0, cp * w00);
-    deposit_xy(sink, x1, y0, y0 * P.scr_w + x1, cp * w10);
-    deposit_xy(sink, x0, y1, y1 * P.scr_w + x0, cp * w01);
-    deposit_xy(sink, x1, y1, y1 * P.scr_w + x1, cp * w11);
+    T.x0 = (uint32_t)floorf(fx);
+    T.y0 = (uint32_t)floorf(fy);
+    T.x1 = T.x0 + 1 < P.scr_w - 1 ? T.x0 + 1 : P.scr_w - 1;
+    T.y1 = T.y0 + 1 < P.scr_h - 1 ? T.y0 + 1 : P.scr_h - 1;
+    float dx = fx - (float)T.x0, dy = fy - (float)T.y0;
+    T.w[0] = (1.0f - dx) * (1.0f - dy);
+    T.w[1] = dx * (1.0f - dy);
+    T.w[2] = (1.0f - dx) * dy;
+    T.w[3] = dx * dy;
+    T.cp = c * power;
+    return T;
+}
+
+__device__ inline Dep screen_texel(const PhotonParams &P, const ScreenTexels &T, int k) {
+    Dep d;
+    d.px = (k & 1) ? T.x1 : T.x0;
+    d.py = (k & 2) ? T.y1 : T.y0;
+    d.idx = d.py * P.scr_w + d.px;
+    d.v = T.cp * T.w[k];
+    d.ok = true;
+    return d;
+}
+
+// Screen::splat (texel_bilinear + 4-texel deposit), lighttrace.rs:65-91.
+template <class Sink>
+__device__ inline void screen_splat(const PhotonParams &P, float s, float r, f3 c, float power, Sink &sink) {
+    const ScreenTexels T = screen_texels(P, s, r, c, power);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const Dep d = screen_texel(P, T, k);
+        deposit_xy(sink, d.px, d.py, d.idx, d.v);
+    }
 }
 
 // Screen::sample, lighttrace.rs:98-113: the bilinear read of texel_bilinear's
@@ -323,6 +352,36 @@ __device__ inline const float *uniform_ptr(const float *p) {
 // Point / legacy estimator deposits for one segment (volume.rs:130-150,
 // 51-64).  ISO: g == 0, where phase_hg(0, c) = 1 / (4 pi * 1^1.5) for every
 // c, bit for bit.  Without a z-buffer every depth passes (+INF).
+// One of a segment's four samples: draws its distance, then projects and
+// weighs it (cam_o = the camera origin, zbuf = the z-buffer or null).
+template <bool LEGACY, bool ISO>
+__device__ inline Dep point_sample(const PhotonParams &P, const PhotonState &st, f3 cmf, float seg_len,
+                                   uint32_t &rng, f3 cam_o, const float *zbuf) {
+    Dep dp;
+    dp.ok = false;
+    float dist = next_f32(rng) * seg_len;
+    f3 p = st.ro + st.rd * dist;
+    float s, t;
+    if (!project(P.proj, p, s, t)) return dp;
+    pix_xy(s, t, P.w, P.h, dp.px, dp.py);
+    dp.idx = dp.py * P.w + dp.px;
+    if (LEGACY) {
+        const float inv4pi = 1.0f / 12.566371f;
+        float contrib = st.power * P.vp.sigma_s * seg_len / 4.0f * inv4pi;
+        dp.v = cmf * contrib;
+    } else {
+        f3 to = cam_o - p;
+        float d = fmaxf(length(to), 1e-3f);
+        if (zbuf && !(d < zbuf[dp.idx])) return dp;
+        float phase = ISO ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
+        float trans = expf(-P.vp.sigma_t * d);
+        float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
+        dp.v = cmf * contrib;
+    }
+    dp.ok = true;
+    return dp;
+}
+
 template <bool LEGACY, bool ISO, class Sink>
 __device__ inline void point_deposits(const PhotonParams &P, const PhotonState &st, f3 cmf, float seg_len,
                                       uint32_t &rng, Sink &sink) {
@@ -330,26 +389,8 @@ __device__ inline void point_deposits(const PhotonParams &P, const PhotonState &
     const float *zbuf = uniform_ptr(P.zbuf);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        float dist = next_f32(rng) * seg_len;
-        f3 p = st.ro + st.rd * dist;
-        float s, t;
-        if (!project(P.proj, p, s, t)) continue;
-        uint32_t px, py;
-        pix_xy(s, t, P.w, P.h, px, py);
-        const uint32_t idx = py * P.w + px;
-        if (LEGACY) {
-            const float inv4pi = 1.0f / 12.566371f;
-            float contrib = st.power * P.vp.sigma_s * seg_len / 4.0f * inv4pi;
-            deposit_xy(sink, px, py, idx, cmf * contrib);
-        } else {
-            f3 to = cam_o - p;
-            float d = fmaxf(length(to), 1e-3f);
-            if (zbuf && !(d < zbuf[idx])) continue;
-            float phase = ISO ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
-            float trans = expf(-P.vp.sigma_t * d);
-            float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
-            deposit_xy(sink, px, py, idx, cmf * contrib);
-        }
+        const Dep d = point_sample<LEGACY, ISO>(P, st, cmf, seg_len, rng, cam_o, zbuf);
+        if (d.ok) deposit_xy(sink, d.px, d.py, d.idx, d.v);
     }
 }
 

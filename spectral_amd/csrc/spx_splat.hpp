@@ -21,7 +21,9 @@ constexpr uint32_t kEmptyKey = 0xFFFFFFFFu;
 // costs only a record, and short probing is cheaper overall (2 measured
 // fastest for the batched photon kernel: fewer divergent compare-and-swap
 // rounds per deposit, slightly more records).
-template <int SLOTS, int PROBES = 8>
+// COUNT_FILL: keep the occupied-slot counter that maybe_flush reads (tables
+// drained only at fixed points skip it: one ballot and LDS atomic per insert).
+template <int SLOTS, int PROBES = 8, bool COUNT_FILL = true>
 struct LdsSplat {
     static_assert((SLOTS & (SLOTS - 1)) == 0, "power of two");
     uint32_t *keys;  // [SLOTS]
@@ -66,9 +68,11 @@ struct LdsSplat {
                 }
             }
         }
-        const uint64_t ins = __ballot(inserted);
-        if (ins && (int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
-            atomicAdd(fill, (uint32_t)__popcll(ins));
+        if (COUNT_FILL) {
+            const uint64_t ins = __ballot(inserted);
+            if (ins && (int)__lane_id() == __ffsll((unsigned long long)__ballot(1)) - 1)
+                atomicAdd(fill, (uint32_t)__popcll(ins));
+        }
         if (slot < 0) return false;
         lds_add((uint32_t)slot, v);
         return true;
@@ -140,6 +144,96 @@ struct HashBinSink {
         uint32_t f = *hash.fill;
         __syncthreads();
         if (f > (uint32_t)(SLOTS / 2)) flush();
+    }
+};
+
+// The record sink of the batched photon kernel, called by every lane of the
+// wave together (converged control flow, each lane with a deposit or
+// none): the wave's open chunk and its fill are then the same on every lane
+// and stay in scalar registers, so an append is a ballot, a prefix count
+// and the lanes' stores -- no leader section, no LDS round trip.  HASH: the
+// LDS table in front (deposit mode 3; a photon past a diffuse bounce skips
+// it).  THREADS: the block size (the drain's trip count is uniform).
+template <int SLOTS, bool HASH, int THREADS>
+struct WaveSink {
+    LdsSplat<SLOTS, kHashBinProbes, false> hash;
+    uint32_t *rkey;
+    float *rval;
+    uint32_t *pool_next;
+    uint32_t *chunk_fill;
+    uint32_t pool_cap;
+    uint32_t pool;                   // this block's first-choice pool
+    uint32_t tile_wlog2;
+    uint32_t chunk = kChunkEmpty;    // the wave's open chunk (first record index)
+    uint32_t fill = 0;               // records written into it
+    // the current item's job
+    uint32_t key_base = 0, width = 1, tiles_x = 1;
+    float *film = nullptr;           // fallback when every pool is full
+
+    __device__ inline void job(uint32_t kb, uint32_t w, float *f) {
+        key_base = kb;
+        width = w;
+        tiles_x = tiles_across(w, tile_wlog2);
+        film = f;
+    }
+
+    __device__ inline void append(bool rec, uint32_t key, uint32_t idx, f3 v) {
+        const uint64_t m = __ballot(rec);
+        if (!m) return;
+        const uint32_t cnt = (uint32_t)__popcll(m);
+        if (chunk != kChunkFull && (chunk == kChunkEmpty || fill + cnt > kBinChunk)) {
+            uint32_t got = 0;
+            if (__lane_id() == 0) {
+                if (chunk != kChunkEmpty) chunk_fill[chunk / kBinChunk] = fill;
+                got = grab_chunk(pool_next, pool_cap, pool);
+            }
+            chunk = __builtin_amdgcn_readfirstlane(got);
+            fill = 0;
+        }
+        if (chunk == kChunkFull) {
+            if (rec) film_add(film, idx, v);
+            return;
+        }
+        if (rec) {
+            const uint32_t at = chunk + fill + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+            rkey[at] = key_base + key;
+            rval[3 * (size_t)at + 0] = v.x;
+            rval[3 * (size_t)at + 1] = v.y;
+            rval[3 * (size_t)at + 2] = v.z;
+        }
+        fill += cnt;
+    }
+
+    // Every lane calls; d.ok = false for no deposit.
+    __device__ inline void deposit(const Dep &d, bool spread) {
+        bool rec = d.ok;
+        if (HASH && d.ok && !spread) rec = !hash.try_add(d.idx, d.v);
+        append(rec, tile_key_xy(d.px, d.py, tiles_x, tile_wlog2), d.idx, d.v);
+    }
+
+    // Whole block must call: the table's sums become records.
+    __device__ inline void flush() {
+        if (!HASH) return;
+        __syncthreads();
+        for (uint32_t k0 = 0; k0 < (uint32_t)SLOTS; k0 += THREADS) {
+            const uint32_t k = k0 + threadIdx.x;
+            const uint32_t key = hash.keys[k];
+            const bool occ = key != kEmptyKey;
+            f3 v = mk(0.f, 0.f, 0.f);
+            if (occ) {
+                v = mk(hash.vals[3 * k], hash.vals[3 * k + 1], hash.vals[3 * k + 2]);
+                hash.keys[k] = kEmptyKey;
+                hash.vals[3 * k] = hash.vals[3 * k + 1] = hash.vals[3 * k + 2] = 0.0f;
+            }
+            append(occ, occ ? tile_key(key, width, tiles_x, tile_wlog2) : 0u, key, v);
+        }
+        __syncthreads();
+    }
+
+    // Every wave calls this once at kernel end.
+    __device__ inline void finish() {
+        if (__lane_id() == 0 && chunk != kChunkEmpty && chunk != kChunkFull)
+            chunk_fill[chunk / kBinChunk] = fill;
     }
 };
 
