@@ -60,6 +60,10 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
 // flushed (the same terms, reassociated: one LDS add per texel instead of
 // three).  A deposit outside the box goes to the film directly.
 constexpr uint32_t kBeamTilePix = 3072;  // 12 KiB: no group of the product image overflows it
+// The tents' weights from the hardware square root and the splat's
+// reciprocal radius (tent_weight): a quarter of the IEEE form's
+// instructions per texel, each weight within about an ulp.
+constexpr bool kFastTent = true;
 
 struct TileSink {
     float *tile;
@@ -135,7 +139,7 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
             if (bx1 < 0) continue;  // no step of the group is on the film
             const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
             if (bw <= 0 || bh <= 0 || bw * bh > (int)kBeamTilePix) {
-                if (ok) splat_transverse(P, sp.s, sp.t, sp.color, sp.radius, direct);
+                if (ok) splat_transverse<GlobalSink, kFastTent>(P, sp.s, sp.t, sp.color, sp.radius, direct);
                 continue;
             }
             const int area = bw * bh;
@@ -143,7 +147,8 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
             __syncthreads();
             if (ok) {
                 TileSink ts{tile, film, cmf, bx0, by0, bw, bh};
-                splat_transverse(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight), sp.radius, ts);
+                splat_transverse<TileSink, kFastTent>(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight),
+                                                      sp.radius, ts);
             }
             __syncthreads();
             for (int e = (int)lane; e < area; e += 64) {
