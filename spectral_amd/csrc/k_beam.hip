@@ -60,9 +60,11 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
 // flushed (the same terms, reassociated: one LDS add per texel instead of
 // three).  A deposit outside the box goes to the film directly.
 constexpr uint32_t kBeamTilePix = 3072;  // 12 KiB: no group of the product image overflows it
-// The tents' weights from the hardware square root and the splat's
-// reciprocal radius (tent_weight): a quarter of the IEEE form's
-// instructions per texel, each weight within about an ulp.
+// The values that scale a splat -- the tent's texel weights (tent_weight:
+// a quarter of the IEEE form's instructions per texel), the step's weight
+// and its radius (beam_step_prep) -- from hardware square roots,
+// reciprocals and exp2, each within a few ulp; where a splat lands stays
+// IEEE-rounded.
 constexpr bool kFastTent = true;
 
 struct TileSink {
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
         for (uint32_t k0 = 0; k0 < m; k0 += 64) {
             const uint32_t k = k0 + lane;
             BeamSplat sp;
-            const bool ok = k < m && beam_step_prep(P, ro, rd, b.w, cmf, ds, k, sp);
+            const bool ok = k < m && beam_step_prep<kFastTent>(P, ro, rd, b.w, cmf, ds, k, sp);
             // the footprint splat_transverse walks (plus its rounded-centre fallback)
             int x0 = 0x7FFFFFFF, x1 = -1, y0 = 0x7FFFFFFF, y1 = -1;
             if (ok) {
@@ -151,9 +153,18 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                                                       sp.radius, ts);
             }
             __syncthreads();
-            for (int e = (int)lane; e < area; e += 64) {
-                const float v = tile[e];
-                if (v != 0.0f) film_add(film, (uint32_t)((by0 + e / bw) * W + bx0 + e % bw), cmf * v);
+            // Flush: lane l takes float k = k0 + l of the box's 3 * area film
+            // floats (pixel k / 3, channel k % 3), so one atomic instruction
+            // covers 64 consecutive floats of a film row (two or three cache
+            // lines) rather than one channel of 64 pixels (six).
+            for (int k0 = 0; k0 < 3 * area; k0 += 64) {
+                const int k = k0 + (int)lane;
+                const int e = k / 3, ch = k - 3 * e;
+                const float v = k < 3 * area ? tile[e] : 0.0f;
+                if (v != 0.0f) {
+                    const float c = ch == 0 ? cmf.x : (ch == 1 ? cmf.y : cmf.z);
+                    unsafeAtomicAdd(film + 3 * (size_t)((by0 + e / bw) * W + bx0 + e % bw) + ch, c * v);
+                }
             }
             __syncthreads();
         }

@@ -409,11 +409,12 @@ __device__ inline void volume_point_segment(const PhotonParams &P, const float4 
 // exactly (sqrt and the division round monotonically and r / r = 1), adds
 // +0 to the weight sum and is not deposited, so each row walks only the
 // span where d2 < r^2 can hold (widened by a pixel against rounding in the
-// span itself) and an exact test, fma(-r, r, d2) >= 0, skips the rest: the
-// same sums and deposits, in the same row-major order, as walking the whole
-// box.  Coordinates fit in 32 bits (the box is clamped to the film).
+// span itself, so the span's half-width may come from the hardware square
+// root, within an ulp) and an exact test, fma(-r, r, d2) >= 0, skips the
+// rest: the same sums and deposits, in the same row-major order, as walking
+// the whole box.  Coordinates fit in 32 bits (the box is clamped to the film).
 __device__ inline void tent_row_span(float cx, float r, float dy, int x0, int x1, int &xs, int &xe) {
-    const float hw = sqrtf(fmaxf(fmaf(-dy, dy, r * r), 0.0f)) + 1.0f;
+    const float hw = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-dy, dy, r * r), 0.0f)) + 1.0f;
     xs = max(x0, (int)floorf(cx - hw));
     xe = min(x1, (int)ceilf(cx + hw));
 }
@@ -505,6 +506,36 @@ struct BeamSplat {
     f3 color;
 };
 
+// The camera projection with hardware reciprocals (v_rcp_f32, 1 ulp) in
+// place of the two IEEE divisions: for quantities that only scale a splat
+// (its radius), not for where a splat lands.
+__device__ inline bool project_approx(const Proj &P, f3 p, float &s, float &t) {
+    f3 dir = p - ld3(P.o);
+    float c = dot(dir, ld3(P.w));
+    if (c >= -1e-6f) return false;
+    float ss = 0.5f + dot(dir, ld3(P.u)) * __builtin_amdgcn_rcpf(-c * P.vw);
+    float tt = 0.5f + dot(dir, ld3(P.v)) * __builtin_amdgcn_rcpf(-c * P.vh);
+    if (ss >= 0.0f && ss < 1.0f && tt >= 0.0f && tt < 1.0f) {
+        s = ss;
+        t = tt;
+        return true;
+    }
+    return false;
+}
+
+// phase_hg (volume.rs:88-92) with x^1.5 = x * sqrt(x) and a hardware
+// reciprocal: within a few ulp.
+__device__ inline float phase_hg_approx(float g, float cos_theta) {
+    float g2 = g * g;
+    float x = fmaxf(1.0f + g2 - 2.0f * g * cos_theta, 1e-6f);
+    return (1.0f - g2) * __builtin_amdgcn_rcpf(12.566371f * (x * __builtin_amdgcn_sqrtf(x)));
+}
+
+// FAST: the step's weight and radius -- the values that scale its splat --
+// from hardware square roots, reciprocals and exp2 (each within a few ulp);
+// its centre (s, t), pixel and camera distance (the z-buffer test) stay
+// IEEE-rounded, so every splat lands where the reference puts it.
+template <bool FAST = false>
 __device__ inline bool beam_step_prep(const PhotonParams &P, f3 ro, f3 rd, float power, f3 cmf, float ds, uint32_t k,
                                       BeamSplat &o) {
     const f3 cam_o = ld3(P.proj.o);
@@ -517,19 +548,30 @@ __device__ inline bool beam_step_prep(const PhotonParams &P, f3 ro, f3 rd, float
     float d = fmaxf(length(to), 1e-3f);
     float zb = P.zbuf ? P.zbuf[idx] : INFINITY;
     if (!(d < zb)) return false;
-    f3 view = to / d;
-    float cos_theta = dot(rd, view);
-    float phase = phase_hg(P.vp.g, cos_theta);
-    float trans = expf(-P.vp.sigma_t * d);
-    float weight = power * P.vp.sigma_s * phase * trans / (d * d) * ds;
+    f3 view;
+    float weight;
+    if (FAST) {
+        const float inv_d = __builtin_amdgcn_rcpf(d);
+        view = to * inv_d;
+        const float phase = phase_hg_approx(P.vp.g, dot(rd, view));
+        const float trans = __expf(-P.vp.sigma_t * d);
+        weight = power * P.vp.sigma_s * phase * trans * (inv_d * inv_d) * ds;
+    } else {
+        view = to / d;
+        float cos_theta = dot(rd, view);
+        float phase = phase_hg(P.vp.g, cos_theta);
+        float trans = expf(-P.vp.sigma_t * d);
+        weight = power * P.vp.sigma_s * phase * trans / (d * d) * ds;
+    }
     f3 perp = cross(view, mk(0.f, 1.f, 0.f));
     if (dot(perp, perp) < 1e-8f) perp = cross(view, mk(1.f, 0.f, 0.f));
-    perp = normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
+    perp = FAST ? perp * (__builtin_amdgcn_rsqf(dot(perp, perp)) * 0.03f)
+                : normalize(perp) * 0.03f;  // R_BEAM, volume.rs:172
     float s2, t2, radius_px;
-    if (project(P.proj, p + perp, s2, t2)) {
+    if (FAST ? project_approx(P.proj, p + perp, s2, t2) : project(P.proj, p + perp, s2, t2)) {
         float dpx = (s2 - s) * (float)P.w;
         float dpy = (t2 - t) * (float)P.h;
-        radius_px = sqrtf(dpx * dpx + dpy * dpy);
+        radius_px = FAST ? __builtin_amdgcn_sqrtf(dpx * dpx + dpy * dpy) : sqrtf(dpx * dpx + dpy * dpy);
         radius_px = radius_px < 1.5f ? 1.5f : (radius_px > 24.0f ? 24.0f : radius_px);
     } else {
         radius_px = 1.0f;
