@@ -78,7 +78,7 @@ struct TileSink {
 // times the texel's weight, with colour = the step's scalar weight).
 __device__ inline void deposit_xy(TileSink &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
     const int x = (int)px - s.bx0, y = (int)py - s.by0;
-    if (x < 0 || x >= s.bw || y < 0 || y >= s.bh) {
+    if ((uint32_t)x >= (uint32_t)s.bw || (uint32_t)y >= (uint32_t)s.bh) {
         film_add(s.film, idx, s.cmf * v.x);
         return;
     }

@@ -408,15 +408,19 @@ __device__ inline void volume_point_segment(const PhotonParams &P, const float4 
 // A texel with dx^2 + dy^2 >= r^2 has weight max(1 - sqrt(d2) / r, 0) = 0
 // exactly (sqrt and the division round monotonically and r / r = 1), adds
 // +0 to the weight sum and is not deposited, so each row walks only the
-// span where d2 < r^2 can hold (widened by a pixel against rounding in the
-// span itself, so the span's half-width may come from the hardware square
-// root, within an ulp) and an exact test, fma(-r, r, d2) >= 0, skips the
-// rest: the same sums and deposits, in the same row-major order, as walking
-// the whole box.  Coordinates fit in 32 bits (the box is clamped to the film).
+// span where d2 < r^2 can hold, |px - cx| < h = sqrt(r^2 - dy^2): the
+// integers in [cx - h', cx + h'] with h' = h + 2^-5 pixel.  The margin
+// covers every rounding on the way: r^2 - dy^2 (r <= 24) is off by at most
+// 2^-14, which moves its square root by at most 2^-7 near zero; the
+// hardware square root adds an ulp; cx -+ h' rounds by at most 2^-8 for
+// films under 2^16 pixels (2^-7 under 2^17).  An exact test,
+// fma(-r, r, d2) >= 0, skips the candidates outside: the same sums and
+// deposits, in the same row-major order, as walking the whole box.
+// Coordinates fit in 32 bits (the box is clamped to the film).
 __device__ inline void tent_row_span(float cx, float r, float dy, int x0, int x1, int &xs, int &xe) {
-    const float hw = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-dy, dy, r * r), 0.0f)) + 1.0f;
-    xs = max(x0, (int)floorf(cx - hw));
-    xe = min(x1, (int)ceilf(cx + hw));
+    const float hw = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-dy, dy, r * r), 0.0f)) + 0.03125f;
+    xs = max(x0, (int)ceilf(cx - hw));
+    xe = min(x1, (int)floorf(cx + hw));
 }
 
 // A texel's tent weight max(1 - sqrt(d2) / r, 0) (volume.rs:198-205).  FAST:
