@@ -469,7 +469,13 @@ __device__ inline void splat_transverse(const PhotonParams &P, float s, float t,
         return;
     }
     const float inv = 1.0f / wsum;
-    for (int py = y0; py <= y1; ++py) {
+    // FAST (the wave-shared LDS tile): each lane starts its deposit rows at
+    // row (lane mod rows) and wraps around, so lanes whose tents sit half a
+    // pixel apart add to different rows in one LDS instruction instead of
+    // to the same texels; the same deposits, in another order.
+    const int nrows = y1 - y0 + 1;
+    int py = y0 + (FAST && nrows > 1 ? (int)(__lane_id() % (uint32_t)nrows) : 0);
+    for (int i = 0; i < nrows; ++i, py = py == y1 ? y0 : py + 1) {
         const float dy = (float)py - cy;
         int xs, xe;
         tent_row_span(cx, r, dy, x0, x1, xs, xe);
