@@ -59,7 +59,14 @@ __global__ __launch_bounds__(256) void k_beam_trace(PhotonParams P, uint32_t sta
 // weight * tent weight, multiplied by cmf once per pixel when it is
 // flushed (the same terms, reassociated: one LDS add per texel instead of
 // three).  A deposit outside the box goes to the film directly.
-constexpr uint32_t kBeamTilePix = 3072;  // 12 KiB: no group of the product image overflows it
+// 5 KiB: 26 one-wave blocks (tile + CIE table) per compute unit, six or
+// seven waves per SIMD; groups whose box is larger are split
+// (k_beam_deposit).  Measured on the product scene's beam pass (3.5e7
+// photons, no z-buffer): 512, 768,
+// 1024, 1280, 1536, 2048, 2560, 3072 pixels gave 3.76, 3.65, 3.62, 3.59,
+// 3.64, 3.83, 3.95, 4.32 s: more resident waves outweigh more flushes.
+constexpr uint32_t kBeamTilePix = 1280;
+constexpr uint32_t kBeamBlocksPerCu = (160u * 1024u) / (kBeamTilePix * 4u + kTblN * 16u);
 // The values that scale a splat -- the tent's texel weights (tent_weight:
 // a quarter of the IEEE form's instructions per texel), the step's weight
 // and its radius (beam_step_prep) -- from hardware square roots,
@@ -98,7 +105,7 @@ __device__ inline int wave_max(int v) {
 // takes its steps 64 at a time, lane k0 + lane; each group of 64 splats
 // into the LDS tile over their footprints' bounding box (as in
 // splat_transverse, volume.rs:178-218), which is then added to the block's
-// film copy; groups whose box exceeds the tile splat straight to the film.
+// film copy.
 __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float4 *segs, const uint32_t *n_segs,
                                                      uint32_t *cursor) {
     __shared__ float4 tbl[kTblN];
@@ -137,36 +144,56 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                 y1 = (int)ceilf(cy + r);
                 y1 = y1 < 0 ? 0 : (y1 > H - 1 ? H - 1 : y1);
             }
-            const int bx0 = wave_min(x0), bx1 = wave_max(x1), by0 = wave_min(y0), by1 = wave_max(y1);
-            if (bx1 < 0) continue;  // no step of the group is on the film
-            const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
-            if (bw <= 0 || bh <= 0 || bw * bh > (int)kBeamTilePix) {
-                if (ok) splat_transverse<GlobalSink, kFastTent>(P, sp.s, sp.t, sp.color, sp.radius, direct);
-                continue;
-            }
-            const int area = bw * bh;
-            for (int e = (int)lane; e < area; e += 64) tile[e] = 0.0f;
-            __syncthreads();
-            if (ok) {
-                TileSink ts{tile, film, cmf, bx0, by0, bw, bh};
-                splat_transverse<TileSink, kFastTent>(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight),
-                                                      sp.radius, ts);
-            }
-            __syncthreads();
-            // Flush: lane l takes float k = k0 + l of the box's 3 * area film
-            // floats (pixel k / 3, channel k % 3), so one atomic instruction
-            // covers 64 consecutive floats of a film row (two or three cache
-            // lines) rather than one channel of 64 pixels (six).
-            for (int k0 = 0; k0 < 3 * area; k0 += 64) {
-                const int k = k0 + (int)lane;
-                const int e = k / 3, ch = k - 3 * e;
-                const float v = k < 3 * area ? tile[e] : 0.0f;
-                if (v != 0.0f) {
-                    const float c = ch == 0 ? cmf.x : (ch == 1 ? cmf.y : cmf.z);
-                    unsafeAtomicAdd(film + 3 * (size_t)((by0 + e / bw) * W + bx0 + e % bw) + ch, c * v);
+            // The group's steps in sub-groups: all 64 when their footprint box
+            // fits the tile, else halves, quarters, ... (a single step's box,
+            // at most 50 x 50 texels for the 24-pixel radius cap, fits any
+            // tile of 2500 pixels or more; below that it splats to the film).
+            uint32_t lo = 0, S = 64;
+            while (lo < 64) {
+                const bool in = lane >= lo && lane < lo + S;
+                const int bx0 = __builtin_amdgcn_readfirstlane(wave_min(in ? x0 : 0x7FFFFFFF));
+                const int bx1 = __builtin_amdgcn_readfirstlane(wave_max(in ? x1 : -1));
+                const int by0 = __builtin_amdgcn_readfirstlane(wave_min(in ? y0 : 0x7FFFFFFF));
+                const int by1 = __builtin_amdgcn_readfirstlane(wave_max(in ? y1 : -1));
+                if (bx1 < 0) {  // no step of the sub-group is on the film
+                    lo += S;
+                    continue;
                 }
+                const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+                if (bw * bh > (int)kBeamTilePix) {
+                    if (S > 1) {
+                        S >>= 1;
+                        continue;
+                    }
+                    if (in && ok) splat_transverse<GlobalSink, kFastTent>(P, sp.s, sp.t, sp.color, sp.radius, direct);
+                    lo += 1;
+                    continue;
+                }
+                const int area = bw * bh;
+                for (int e = (int)lane; e < area; e += 64) tile[e] = 0.0f;
+                __syncthreads();
+                if (in && ok) {
+                    TileSink ts{tile, film, cmf, bx0, by0, bw, bh};
+                    splat_transverse<TileSink, kFastTent>(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight),
+                                                          sp.radius, ts);
+                }
+                __syncthreads();
+                // Flush: lane l takes float f = f0 + l of the box's 3 * area
+                // film floats (pixel f / 3, channel f % 3), so one atomic
+                // instruction covers 64 consecutive floats of a film row (two
+                // or three cache lines) rather than one channel of 64 pixels.
+                for (int f0 = 0; f0 < 3 * area; f0 += 64) {
+                    const int f = f0 + (int)lane;
+                    const int e = f / 3, ch = f - 3 * e;
+                    const float v = f < 3 * area ? tile[e] : 0.0f;
+                    if (v != 0.0f) {
+                        const float cc = ch == 0 ? cmf.x : (ch == 1 ? cmf.y : cmf.z);
+                        unsafeAtomicAdd(film + 3 * (size_t)((by0 + e / bw) * W + bx0 + e % bw) + ch, cc * v);
+                    }
+                }
+                lo += S;
+                __syncthreads();
             }
-            __syncthreads();
         }
     }
 }
@@ -187,9 +214,8 @@ hipError_t run_beam_wavefront(spx_ctx *ctx, const PhotonParams &p, void *scratch
         hipLaunchKernelGGL(k_beam_trace, dim3((count + 255) / 256), dim3(256), 0, s, p, start, count, segs,
                            counters);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        // one-wave blocks of 13 KiB LDS (tile + CIE table): twelve resident
-        // per compute unit
-        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * 12), dim3(64), 0, s, p, segs, counters, counters + 1);
+        // one-wave blocks, as many as the compute units' LDS holds
+        hipLaunchKernelGGL(k_beam_deposit, dim3(cus * kBeamBlocksPerCu), dim3(64), 0, s, p, segs, counters, counters + 1);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 FILM_TOL = 5e-3   # relative L1 per XYZ channel
 PATH_RTOL = 1e-6  # per-photon states, relative
-BEAM_TILE_PIX = 3072  # the wavefront beam kernel's LDS tile (kBeamTilePix, k_beam.hip)
+BEAM_TILE_PIX = 1280  # the wavefront beam kernel's LDS tile (kBeamTilePix, k_beam.hip)
 
 
 def test_rng_parity(ctx, oracle):
@@ -439,17 +439,20 @@ def test_beam_wavefront_groups_larger_than_the_tile(ctx, oracle):
     # a camera 0.15-1.35 units above the DSOTM beam sheet, looking down on
     # it: tent boxes of 11-50 pixels a side (4 % of the steps 45 or more, at
     # the 24 px clamp; the oracle's counts), so the boxes of the groups of 64
-    # steps with the largest tents exceed the LDS tile (3072 pixels) and
-    # those groups splat straight to the film; the film still equals the
-    # per-photon form to summation order and the oracle
+    # steps with the largest tents exceed the LDS tile (1280 pixels): those
+    # groups are split into halves, quarters, ... until each part fits, and
+    # single steps whose own box exceeds the tile splat straight to the film;
+    # the film still equals the per-photon form to summation order and the
+    # oracle
     sc, bm, _ = scenes.dsotm_scene()
     cam = S.Camera.look_at((-1.93, -0.52, 0.75), (-1.93, -0.52, 0.0), (0, 1, 0), 60.0, 1.0)
     vp = A.VolParams(0.5, 0.06, 0.5, 14.0)
     w = h = 256
     # the oracle's own steps: some group of 64 has a box above the tile
     box, _ = oracle.volume_beam_max_box(sc, cam, bm, w, h, 200, 0, vp, None, 5, opts(A.ILLUM_E), 64)
-    print("largest 64-step box", box, "pixels")
-    assert box > BEAM_TILE_PIX
+    box1, _ = oracle.volume_beam_max_box(sc, cam, bm, w, h, 200, 0, vp, None, 5, opts(A.ILLUM_E), 1)
+    print("largest 64-step box", box, "pixels, largest single-step box", box1)
+    assert box > BEAM_TILE_PIX and box1 > BEAM_TILE_PIX
     g, c = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM, engine=A.ENGINE_WAVEFRONT)
     p, _ = _volume_pair(ctx, oracle, sc, cam, bm, w, h, 200, vp, 5, est=A.EST_BEAM)
     err, err_p = rel_l1_per_channel(g, c), rel_l1_per_channel(g, p)
