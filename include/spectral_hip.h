@@ -178,9 +178,10 @@ typedef struct {
                                 (0 = 4 per compute unit) */
     uint32_t chunk_paths;    /* batched engine: camera work item, in paths (default 16384) */
     uint32_t chunk_photons;  /* batched engine: photon work item, in photons (default 8192) */
-    uint32_t overlap_bins;   /* batched engine, binned deposits: 1 = double-buffered rounds, the bin
-                                passes of round r on a second stream while round r+1 traces
-                                (two record buffers); default 0 */
+    uint32_t overlap_bins;   /* batched engine, binned deposits: N = 1 or 2 overlapped rounds, the
+                                bin passes of round r on a second stream while rounds r+1 .. r+N
+                                trace (N + 1 record buffers, round r+N+1 reuses round r's);
+                                default 0 */
     uint32_t camera_blocks;  /* batched engine: 0 = the camera round runs first, on the whole grid;
                                 N > 0 = it runs on its own stream with N blocks, concurrently with
                                 the photon rounds; default 0 */

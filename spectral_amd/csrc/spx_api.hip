@@ -466,7 +466,7 @@ int spx_ctx_set_tuning(spx_ctx *ctx, const spx_tuning *t) {
         t->chunk_log2 > 26 || t->replica_mib > 65536 || t->bin_log2 < 15 || t->bin_log2 > 30 ||
         t->tile_wlog2 < kTileLog2Min || t->tile_wlog2 > kTileLog2Max || t->batch_blocks > 65536 ||
         t->chunk_paths < 64 || t->chunk_paths > (1u << 22) || t->chunk_photons < 64 ||
-        t->chunk_photons > (1u << 24) || t->overlap_bins > 1 || t->camera_blocks > 65536 ||
+        t->chunk_photons > (1u << 24) || t->overlap_bins >= (uint32_t)kRoundBufs || t->camera_blocks > 65536 ||
         (t->photon_threads != 256 && t->photon_threads != 1024))
         return set_error(ctx, SPX_EINVAL, "spx_ctx_set_tuning: value out of range");
     ctx->tuning = *t;
@@ -491,7 +491,7 @@ void spx_ctx_destroy(spx_ctx *ctx) {
     hipDeviceSynchronize();
     if (ctx->scratch) hipFree(ctx->scratch);
     if (ctx->replica) hipFree(ctx->replica);
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < kRoundBufs; ++k)
         if (ctx->bins[k]) hipFree(ctx->bins[k]);
     if (ctx->aux) hipStreamDestroy(ctx->aux);
     if (ctx->aux_cam) hipStreamDestroy(ctx->aux_cam);

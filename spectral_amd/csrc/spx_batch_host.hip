@@ -32,7 +32,8 @@ namespace {
 struct Schedule {
     uint32_t splat_mode = 2, blocks = 0, n_rounds = 0, n_jobs = 0;
     bool binned = false;
-    bool overlap = false;  // double-buffered photon rounds (bin passes on a second stream)
+    bool overlap = false;  // overlapped photon rounds (bin passes on a second stream)
+    uint32_t nbuf = 1;     // record buffers the rounds cycle through (overlapped: 2 or 3)
     uint32_t cam_blocks = 0;  // > 0: the camera round runs concurrently, on its own stream
     uint32_t ph_threads = 256;  // photon-kernel block size (1024: one block per CU)
     std::vector<BatchJob> table;
@@ -220,6 +221,7 @@ int plan_schedule(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, const spx_
         S.n_rounds = r + 1;
         S.bin_want = (size_t)(most / kBinFill);
         S.overlap = T.overlap_bins != 0 && S.n_rounds > 2;
+        S.nbuf = S.overlap ? std::min<uint32_t>(T.overlap_bins + 1u, (uint32_t)kRoundBufs) : 1u;
     }
 
     // Film replicas for the photon jobs (atomic deposit modes only).
@@ -336,12 +338,12 @@ hipError_t timed(spx_ctx *ctx, bool timing, uint32_t kind, hipStream_t s, F laun
     return e;
 }
 
-// Deposit-round buffers of one launch: bst[1] / shared[1] are used only by
-// overlapped schedules; `ev` holds 2 ordering events per photon round and
-// `s2` is the second stream.
+// Deposit-round buffers of one launch: bst[k] / shared[k] for k >= 1 are used
+// only by overlapped schedules; `ev` holds 2 ordering events per photon round
+// and `s2` is the second stream.
 struct RoundBufs {
-    BinState bst[2];
-    const uint32_t *shared[2] = {nullptr, nullptr};
+    BinState bst[kRoundBufs];
+    const uint32_t *shared[kRoundBufs] = {};
     hipStream_t s2 = nullptr;
     hipEvent_t *ev = nullptr;   // overlapped rounds: 2 per photon round
     hipStream_t s3 = nullptr;   // concurrent camera round
@@ -358,13 +360,18 @@ uint32_t photon_rounds(const Schedule &S) {
 // The kernel sequence of a schedule whose image is at `d`: claim counters
 // reset, replica clear, the camera round, then per photon round the bin
 // reset, the trace kernel and the bin passes, and finally the replica sums.
-// Overlapped schedules alternate two record buffers: round k's bin passes
-// run on s2 while round k+1 traces on s; round k+2 reuses k's buffer only
-// after k's passes end, and the passes stay in round order on s2 (they add
-// into the same films).  Only kernel launches and event ordering (no memset
-// or copy), so the sequence can be captured into a graph.
+// Overlapped schedules cycle through S.nbuf record buffers: round k's bin
+// passes run on s2 while rounds k+1 .. k+nbuf-1 trace on s; round k+nbuf
+// reuses k's buffer only after k's passes (and the buffer's reset behind
+// them) end, and the passes stay in round order on s2 (they add into the
+// same films).  Beside a photon kernel that holds most of the GPU the passes
+// take longer than a round's trace, so with two buffers each round waited
+// for the passes of the round before last (4-8 ms a round on the 512-job
+// stream); three take that wait off.  Only kernel launches and event
+// ordering (no memset or copy), so the sequence can be captured into a graph.
 hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B, float *replica, hipStream_t s,
                    bool timing) {
+    const uint32_t n_photon_rounds = photon_rounds(S);
     hipError_t e = launch_queues_reset((BatchQueues *)d, S.n_rounds, S.qstride, s);
     if (e == hipSuccess && !S.reps.empty()) e = launch_fill(replica, 0.0f, S.rep_total, s);
     const uint32_t acc_blocks = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256) * 4u;
@@ -397,16 +404,21 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
                           [&] { return launch_bins_resolve(B.bst[0], B.shared[0], acc_blocks, s); });
             continue;
         }
-        const uint32_t buf = k & 1u;
+        const uint32_t buf = k % S.nbuf;
         hipEvent_t traced = B.ev[2 * k], resolved = B.ev[2 * k + 1];
-        if (k >= 2) e = hipStreamWaitEvent(s, B.ev[2 * (k - 2) + 1], 0);  // buffer free again
-        if (e == hipSuccess) e = launch_bins_reset(B.bst[buf], s);
+        if (k >= S.nbuf)  // the buffer's previous round resolved (and reset it)
+            e = hipStreamWaitEvent(s, B.ev[2 * (k - S.nbuf) + 1], 0);
+        else  // the buffer's first round of the launch
+            e = launch_bins_reset(B.bst[buf], s);
         if (e == hipSuccess) e = trace(r, B.bst[buf]);
         if (e == hipSuccess) e = hipEventRecord(traced, s);
         if (e == hipSuccess) e = hipStreamWaitEvent(B.s2, traced, 0);
         if (e == hipSuccess)
             e = timed(ctx, timing, SPX_KT_BINS, B.s2,
                       [&] { return launch_bins_resolve(B.bst[buf], B.shared[buf], acc_blocks, B.s2); });
+        // The round that reuses the buffer finds it reset: the reset runs
+        // behind the passes on s2, not on the trace stream's critical path.
+        if (e == hipSuccess && k + S.nbuf < n_photon_rounds) e = launch_bins_reset(B.bst[buf], B.s2);
         if (e == hipSuccess) e = hipEventRecord(resolved, B.s2);
         ++k;
     }
@@ -506,7 +518,7 @@ struct spx_batch_plan {
     spx::Schedule S;
     char *dev = nullptr;   // schedule image
     char *host = nullptr;  // pinned copy of the image
-    char *bins[2] = {nullptr, nullptr};  // deposit round buffers (second: overlapped rounds)
+    char *bins[kRoundBufs] = {};  // deposit round buffers (more than one: overlapped rounds)
     spx::BinLayout bl;
     spx::RoundBufs rb;
     float *replica = nullptr;
@@ -547,7 +559,7 @@ static void plan_free(spx_batch_plan *p) {
 // Stream-ordered upload of the job map into every deposit buffer of a plan.
 static int plan_upload_maps(spx_batch_plan *p, hipStream_t s) {
     if (!p->S.binned) return SPX_OK;
-    for (int k = 0; k < (p->S.overlap ? 2 : 1); ++k) {
+    for (uint32_t k = 0; k < p->S.nbuf; ++k) {
         int rc = bins_upload_map(p->ctx, p->bins[k], p->bl, p->S.map, s);
         if (rc) return rc;
     }
@@ -588,7 +600,7 @@ static int plan_upload(spx_batch_plan *p, hipStream_t s) {
 static void plan_bind(spx_batch_plan *p) {
     memset(&p->rb.bst, 0, sizeof p->rb.bst);
     if (!p->S.binned) return;
-    for (int k = 0; k < (p->S.overlap ? 2 : 1); ++k) {
+    for (uint32_t k = 0; k < p->S.nbuf; ++k) {
         bins_bind(p->bins[k], p->bl, p->S.map, p->rb.bst[k], &p->rb.shared[k]);
         p->rb.bst[k].atomic_film = p->S.overlap ? 1u : 0u;
     }
@@ -645,7 +657,7 @@ static int ctx_overlap_streams(spx_ctx *ctx, const Schedule &S, RoundBufs &B) {
 static int ctx_bins(spx_ctx *ctx, const Schedule &S, hipStream_t s, RoundBufs &B) {
     memset(&B.bst, 0, sizeof B.bst);
     if (!S.binned) return SPX_OK;
-    for (int k = 0; k < (S.overlap ? 2 : 1); ++k) {
+    for (uint32_t k = 0; k < S.nbuf; ++k) {
         int rc = prepare_bins(ctx, S.map, S.bin_want, s, B.bst[k], &B.shared[k], k);
         if (rc) return rc;
         B.bst[k].atomic_film = S.overlap ? 1u : 0u;
@@ -673,8 +685,7 @@ int spx::batch_plan_create_shared(spx_ctx *ctx, const spx_job *jobs, uint32_t n_
     }
     if (S.binned) {
         p->bl = bins_layout(S.map, S.bin_want, ctx->tuning.bin_log2);
-        p->bins[0] = (char *)ctx->bins[0];
-        p->bins[1] = (char *)ctx->bins[1];
+        for (int k = 0; k < kRoundBufs; ++k) p->bins[k] = (char *)ctx->bins[k];
     }
     plan_bind(p);  // the maps were uploaded by prepare_bins
     if (!S.reps.empty()) {
@@ -765,7 +776,7 @@ int spx_batch_plan_create(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, co
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->uploaded, hipEventDisableTiming);
     if (e == hipSuccess && S.binned) {
         p->bl = bins_layout(S.map, S.bin_want, ctx->tuning.bin_log2);
-        for (int k = 0; k < (S.overlap ? 2 : 1) && e == hipSuccess; ++k) e = hipMalloc(&p->bins[k], p->bl.total);
+        for (uint32_t k = 0; k < S.nbuf && e == hipSuccess; ++k) e = hipMalloc(&p->bins[k], p->bl.total);
     }
     if (e == hipSuccess && !S.reps.empty()) {
         e = hipMalloc(&p->replica, S.rep_total * sizeof(float));
@@ -906,7 +917,7 @@ int spx_autotune(spx_ctx *ctx, const spx_job *jobs, uint32_t n_jobs, uint32_t sw
     // with), then the work-item and tile sizes.
     const std::vector<Axis> axes = {
         {&spx_tuning::splat_mode, {2, 3, 1}, false},
-        {&spx_tuning::overlap_bins, {0, 1}, true},
+        {&spx_tuning::overlap_bins, {0, 1, 2}, true},
         {&spx_tuning::camera_blocks, {0, cus / 2, cus, cus * 2}, false},
         {&spx_tuning::batch_blocks, {cus * 2, cus * 3, cus * 4, cus * 8}, false},
         {&spx_tuning::chunk_photons, {2048, 4096, 8192, 16384, 32768}, false},

@@ -12,6 +12,9 @@
 
 namespace spx {
 
+// Record buffers of overlapped deposit rounds at most (spx_tuning.overlap_bins + 1).
+constexpr int kRoundBufs = 3;
+
 // Per-launch parameter blocks (passed by value as kernel arguments).
 struct CamParams {
     spx_camera cam;
@@ -62,8 +65,8 @@ struct spx_ctx {
     float *replica = nullptr;     // grow-only film replica buffer
     size_t replica_floats = 0;
     spx_tuning tuning;
-    void *bins[2] = {nullptr, nullptr};  // grow-only binned-deposit buffers (2nd: overlapped rounds)
-    size_t bins_bytes[2] = {0, 0};
+    void *bins[spx::kRoundBufs] = {};  // grow-only binned-deposit buffers (more than one: overlapped rounds)
+    size_t bins_bytes[spx::kRoundBufs] = {};
     hipStream_t aux = nullptr;           // second stream of overlapped bin passes
     hipStream_t aux_cam = nullptr;       // stream of a concurrent camera round
     std::vector<hipEvent_t> sync_ev;     // cross-stream ordering events (no timing)

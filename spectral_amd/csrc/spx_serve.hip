@@ -42,12 +42,12 @@ struct Req {
 
 // Deposit queues are shared by the requests of a step and never exceed the
 // round buffers' record capacity (binned deposits: records + padded sorted
-// copy; two buffers when rounds overlap), so the requests' queue shares are
+// copy; one more buffer per overlapped round), so the requests' queue shares are
 // charged up to that bound.
 uint64_t queue_cap(const spx_ctx *ctx) {
     if (ctx->tuning.splat_mode < 2) return UINT64_MAX;
     return (uint64_t)(kRoundBytesPerRecord * (double)((uint64_t)1 << ctx->tuning.bin_log2)) *
-           (ctx->tuning.overlap_bins ? 2u : 1u);
+           (ctx->tuning.overlap_bins + 1u);
 }
 
 size_t job_film_floats(const spx_job &j) {

@@ -27,10 +27,11 @@ from .core import Context, DeviceScene, Film, _check, opts
 # spx_autotune's pick on the 512-job mixed stream (bench.py --autotune, one
 # coordinate sweep; repeated runs pick 768 or 1024 persistent blocks and
 # 4096- or 8192-photon items within 2 % of each other): deposit mode 3,
-# double-buffered rounds, a 128-block concurrent camera round.
+# three record buffers cycling over overlapped rounds (256 ms per step
+# against 267 ms with two), a 128-block concurrent camera round.
 TUNED_MIXED512 = {"max_replicas": 16, "replica_mib": 64, "splat_mode": 3, "chunk_log2": 22, "bin_log2": 30,
                   "tile_wlog2": 5, "batch_blocks": 768, "chunk_paths": 16384, "chunk_photons": 4096,
-                  "overlap_bins": 1, "camera_blocks": 128, "photon_threads": 256}
+                  "overlap_bins": 2, "camera_blocks": 128, "photon_threads": 256}
 
 
 @dataclass

@@ -52,16 +52,18 @@ def test_batch_equals_alone(ctx, splat_mode, bin_log2, threads):
 
 @pytest.mark.parametrize("splat_mode", [2, 3])
 def test_overlapped_rounds_equal_sequential(ctx, splat_mode):
-    # Double-buffered deposit rounds (bin passes of round k on a second
-    # stream while round k+1 traces) give the films of sequential rounds;
+    # Overlapped deposit rounds (bin passes of round k on a second stream
+    # while the next rounds trace, two or three record buffers) give the
+    # films of sequential rounds;
     # bin_log2 = 15 forces many rounds and pool overflow into film atomics.
     specs = _small_mix()
     br = jobs.BatchRunner(ctx, specs)
     old = ctx.tuning()
     try:
         films = []
-        # sequential; overlapped rounds; overlapped rounds + a concurrent camera round
-        for overlap, cam_blocks in ((0, 0), (1, 0), (1, 256)):
+        # sequential; overlapped rounds; overlapped rounds + a concurrent camera round;
+        # three record buffers
+        for overlap, cam_blocks in ((0, 0), (1, 0), (1, 256), (2, 256)):
             ctx.set_tuning(splat_mode=splat_mode, bin_log2=15, overlap_bins=overlap, camera_blocks=cam_blocks)
             br.clear()
             br.run_batched()

@@ -24,10 +24,10 @@ def _images(br):
 
 
 @pytest.mark.parametrize("splat_mode,overlap,threads", [(1, 0, 256), (2, 0, 256), (3, 0, 256), (2, 1, 256),
-                                                         (3, 1, 256), (3, 1, 1024)])
+                                                         (3, 1, 256), (3, 1, 1024), (2, 2, 256), (3, 2, 256)])
 def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap, threads):
-    # overlap = 1: the graph forks the bin passes onto a second branch
-    # (double-buffered rounds; bin_log2 = 21 makes several rounds without
+    # overlap = 1 / 2: the graph forks the bin passes onto a second branch
+    # (two / three record buffers; bin_log2 = 21 makes several rounds without
     # overflowing the record pools, whose fallback atomics would reorder sums);
     # threads = 1024: the photon kernel in compute-unit-wide blocks
     specs = _small_mix()
@@ -37,7 +37,7 @@ def test_plan_replays_equal_batch_run(ctx, splat_mode, overlap, threads):
     old = ctx.tuning()
     ctx.set_tuning(photon_threads=threads)
     if overlap:  # and the camera round on a third branch
-        ctx.set_tuning(overlap_bins=1, bin_log2=21, camera_blocks=256)
+        ctx.set_tuning(overlap_bins=overlap, bin_log2=21, camera_blocks=256)
     try:
         br.clear()
         br.run_batched(cfg=cfg)
