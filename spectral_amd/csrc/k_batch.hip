@@ -103,19 +103,67 @@ __device__ inline void batch_photon_item(const SceneData &S, const float4 *tbl, 
 // photon's life of each other (a fixed stride of photons per lane leaves
 // the wave waiting on its longest lane).  A photon's result depends only on
 // its index, not on the lane that traces it.
-template <class WS>
+//
+// POOL: the few lanes that finish in one bounce (about 13 of 64) would each
+// compute their next photon's start -- its draws, table lookups and the fp64
+// Sellmeier index of the scene's glass -- with the rest of the wave idle.
+// Instead the wave keeps the starts of its next photons in a pool (`pool`:
+// kPhotonStartWords x 64 words of LDS, photon p's start at slot p % 64):
+// when the finishing lanes need photons the pool does not hold, the wave
+// computes the starts of all its next photons up to 64 ahead at once, one
+// per lane, and a finishing lane then reads its photon's start back.
+constexpr uint32_t kPoolSlots = 64;
+constexpr uint32_t kPoolWords = kPhotonStartWords * kPoolSlots;
+constexpr uint32_t kPhotonPoolBytes = 4 * kPoolWords * 4;  // four waves of 256-thread blocks
+
+__device__ inline void pool_store(float *pool, uint32_t slot, const PhotonStart &s) {
+    pool[0 * kPoolSlots + slot] = __uint_as_float(s.rng);
+    pool[1 * kPoolSlots + slot] = s.a;
+    pool[2 * kPoolSlots + slot] = s.b;
+    pool[3 * kPoolSlots + slot] = s.u;
+    pool[4 * kPoolSlots + slot] = s.power;
+    pool[5 * kPoolSlots + slot] = s.n;
+    pool[6 * kPoolSlots + slot] = s.cmf.x;
+    pool[7 * kPoolSlots + slot] = s.cmf.y;
+    pool[8 * kPoolSlots + slot] = s.cmf.z;
+}
+
+__device__ inline PhotonStart pool_load(const float *pool, uint32_t slot) {
+    PhotonStart s;
+    s.rng = __float_as_uint(pool[0 * kPoolSlots + slot]);
+    s.a = pool[1 * kPoolSlots + slot];
+    s.b = pool[2 * kPoolSlots + slot];
+    s.u = pool[3 * kPoolSlots + slot];
+    s.power = pool[4 * kPoolSlots + slot];
+    s.n = pool[5 * kPoolSlots + slot];
+    s.cmf = mk(pool[6 * kPoolSlots + slot], pool[7 * kPoolSlots + slot], pool[8 * kPoolSlots + slot]);
+    return s;
+}
+
+// Orders one wave's LDS stores before its later LDS loads of other lanes'
+// words (the LDS executes a wave's instructions in order; this keeps the
+// compiler from moving the loads up).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool POOL, class WS>
 __device__ inline void conv_photon_item(const SceneData &S, const float4 *tbl, const PhotonParams &P,
-                                        uint32_t kind, uint32_t start, uint32_t count, WS &sink) {
+                                        uint32_t kind, uint32_t start, uint32_t count, WS &sink, float *pool) {
     const uint32_t waves = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = __lane_id();
     const uint32_t per = (count + waves - 1) / waves;
     const uint32_t w0 = min(wave * per, count), w1 = min(w0 + per, count);
     uint32_t next = w0 + 64;  // the share's next photon not yet handed out (wave-uniform)
+    uint32_t filled = next;   // POOL: the pool holds the starts of photons [next, filled)
+    const uint32_t g0 = POOL ? uniform_u32(scene_first_glass(S)) : kNoGlass;
     uint32_t f = w0 + lane;
     bool live = f < w1;
     bool spread = false;  // the photon has scattered diffusely (its deposits skip the LDS table)
     uint32_t b = 0;
     PhotonState st;
-    if (live) photon_begin(P, tbl, P.photon_base + start + f, st);
+    if (live) photon_resume(P, photon_start(P, tbl, P.photon_base + start + f, g0), g0, st);
     const bool screen = kind == SPX_JOB_LIGHTTRACE;
     const bool legacy = uniform_u32(P.estimator) == SPX_EST_LEGACY;
     const bool iso = uniform_f32(P.vp.g) == 0.0f;
@@ -177,12 +225,28 @@ __device__ inline void conv_photon_item(const SceneData &S, const float4 *tbl, c
         const uint64_t m = __ballot(done);
         if (m) {
             const uint32_t mine = next + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            if (POOL && min(next + (uint32_t)__popcll(m), w1) > filled) {
+                // (wave-uniform) the starts of photons [filled, next + 64), the pool's free slots
+                const uint32_t nf = min(next + kPoolSlots, w1) - filled;
+                if (lane < nf) {
+                    const uint32_t p = filled + lane;
+                    pool_store(pool, p % kPoolSlots, photon_start(P, tbl, P.photon_base + start + p, kNoGlass));
+                }
+                if (g0 != kNoGlass && lane < nf) {  // the fp64 index apart: fewer registers live at once
+                    const uint32_t slot = (filled + lane) % kPoolSlots;
+                    const float lambda = kLambdaMin + kLambdaRange * pool[3 * kPoolSlots + slot];
+                    pool[5 * kPoolSlots + slot] = glass_n(g0, P.dispersion ? lambda : 550.0f);
+                }
+                filled += nf;
+                wave_lds_sync();
+            }
             next += (uint32_t)__popcll(m);
             if (done) {
                 f = mine;
                 live = f < w1;
                 if (live) {
-                    photon_begin(P, tbl, P.photon_base + start + f, st);
+                    if (POOL) photon_resume(P, pool_load(pool, f % kPoolSlots), g0, st);
+                    else photon_begin(P, tbl, P.photon_base + start + f, st);
                     spread = false;
                     b = 0;
                 }
@@ -263,16 +327,23 @@ __global__ __launch_bounds__(256, 4) void k_batch_camera(const BatchJob *__restr
 // threads; 8192 at 1024 threads, one block filling a compute unit, so all of
 // the unit's photons share one 128 KiB table).  Capped at 128 VGPRs (four
 // waves per SIMD), so a concurrent camera round's waves still fit beside
-// three photon blocks per compute unit.
+// three photon blocks per compute unit.  The binned forms of 256 threads
+// take each wave's next photons from a pool of precomputed starts
+// (conv_photon_item) in 9 KiB of dynamic LDS (kPhotonPoolBytes at launch),
+// which still leaves a camera block room beside three photon blocks; being
+// dynamic, it does not lower the four waves per SIMD the register budget is
+// planned for.
 template <int SPLAT, int THREADS = 256>
 __global__ __launch_bounds__(THREADS, 4) void k_batch_photons(const BatchJob *__restrict__ jobs,
                                                            const BatchItem *__restrict__ items, BatchQueues *q,
                                                            BinState bins) {
     constexpr int kSlots = THREADS / 256 * kBatchSlots;
     constexpr int kWaves = THREADS / 64;
+    constexpr bool kPool = SPLAT >= 2 && THREADS == 256;
     __shared__ BatchBlock B;
     __shared__ uint32_t sk[SPLAT == 1 || SPLAT == 3 ? kSlots : 1];
     __shared__ float sv[SPLAT == 1 || SPLAT == 3 ? 3 * kSlots : 1];
+    extern __shared__ float spool[];  // kPool: kWaves * kPoolWords (the launch's dynamic LDS)
     __shared__ uint32_t sfill;
     LdsSplat<kSlots, SPLAT == 3 ? kHashBinProbes : 8, false> hash{sk, sv, &sfill, nullptr};
     if (threadIdx.x == 0) {
@@ -299,7 +370,8 @@ __global__ __launch_bounds__(THREADS, 4) void k_batch_photons(const BatchJob *__
             batch_photon_item(B.S, B.tbl, P, kind, item.start, item.count, g);
         } else if constexpr (SPLAT >= 2) {
             ws.job(P.key_base, kind == SPX_JOB_LIGHTTRACE ? P.scr_w : P.w, P.film);
-            conv_photon_item(B.S, B.tbl, P, kind, item.start, item.count, ws);
+            conv_photon_item<kPool>(B.S, B.tbl, P, kind, item.start, item.count, ws,
+                                    kPool ? spool + (threadIdx.x >> 6) * kPoolWords : nullptr);
             ws.flush();
         } else {
             hash.film = deposit_film(P);
@@ -341,9 +413,9 @@ hipError_t launch_batch(bool camera, const BatchJob *jobs, const BatchItem *item
     else if (splat_mode == 1)
         hipLaunchKernelGGL(k_batch_photons<1>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
     else if (splat_mode == 2)
-        hipLaunchKernelGGL(k_batch_photons<2>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+        hipLaunchKernelGGL(k_batch_photons<2>, dim3(blocks), dim3(256), kPhotonPoolBytes, s, jobs, items, q, bins);
     else
-        hipLaunchKernelGGL(k_batch_photons<3>, dim3(blocks), dim3(256), 0, s, jobs, items, q, bins);
+        hipLaunchKernelGGL(k_batch_photons<3>, dim3(blocks), dim3(256), kPhotonPoolBytes, s, jobs, items, q, bins);
     return hipGetLastError();
 }
 

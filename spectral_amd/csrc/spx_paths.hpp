@@ -157,25 +157,61 @@ __device__ inline f3 photon_hero(const PhotonParams &P, const float4 *tbl, float
     return mk(c.x * s, c.y * s, c.z * s);
 }
 
+// A photon's start in two parts: photon_start holds its costly terms (the
+// RNG key and first three draws, the wavelength's table lookups and, for
+// the scene's first glass g0, Glass::n -- the value photon_n would compute
+// at the photon's first hit of g0), so that a wave can compute the starts
+// of its next photons together with every lane busy; photon_resume builds
+// the state from them.  photon_begin is the two in a row.
+struct PhotonStart {
+    uint32_t rng;  // after the three draws
+    float a, b, u;
+    float power, n;
+    f3 cmf;
+};
+constexpr uint32_t kPhotonStartWords = 9;
+
+__device__ inline PhotonStart photon_start(const PhotonParams &P, const float4 *tbl, uint32_t i, uint32_t g0) {
+    PhotonStart s;
+    s.rng = rng_new(i, P.seed);
+    s.a = next_f32(s.rng);
+    s.b = next_f32(s.rng);
+    s.u = next_f32(s.rng);
+    const float lambda = kLambdaMin + kLambdaRange * s.u;  // = the comb's hero lane, hero_lambda(u, 0)
+    if (P.hero) {
+        s.power = 1.0f;
+        s.cmf = photon_comb(P, tbl, s.u);
+    } else {
+        s.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, lambda, false) * P.ill_scale;
+        s.cmf = cmf_lookup(tbl, lambda);  // the wavelength is fixed: looked up once
+    }
+    s.n = g0 != kNoGlass ? glass_n(g0, P.dispersion ? lambda : 550.0f) : 1.0f;
+    return s;
+}
+
+__device__ inline void photon_resume(const PhotonParams &P, const PhotonStart &s, uint32_t g0, PhotonState &st) {
+    st.rng = s.rng;
+    st.ro = ld3(P.beam.corner) + ld3(P.beam.u) * s.a + ld3(P.beam.v) * s.b;
+    st.u = s.u;
+    st.lambda = kLambdaMin + kLambdaRange * s.u;
+    st.rd = ld3(P.beam.dir);
+    st.n_glass = g0;
+    st.n = s.n;
+    st.comb = P.hero;
+    st.power = s.power;
+    st.cmf = s.cmf;
+}
+
 __device__ inline void photon_begin(const PhotonParams &P, const float4 *tbl, uint32_t i,
                                     PhotonState &st) {
-    st.rng = rng_new(i, P.seed);
-    float a = next_f32(st.rng);
-    float b = next_f32(st.rng);
-    st.ro = ld3(P.beam.corner) + ld3(P.beam.u) * a + ld3(P.beam.v) * b;
-    const float u = next_f32(st.rng);
-    st.u = u;
-    st.lambda = kLambdaMin + kLambdaRange * u;  // = the comb's hero lane, hero_lambda(u, 0)
-    st.rd = ld3(P.beam.dir);
-    st.n_glass = kNoGlass;
-    st.comb = P.hero;
-    if (P.hero) {
-        st.power = 1.0f;
-        st.cmf = photon_comb(P, tbl, u);
-    } else {
-        st.power = P.illuminant == SPX_ILLUM_E ? 1.0f : ill_lookup(tbl, st.lambda, false) * P.ill_scale;
-        st.cmf = cmf_lookup(tbl, st.lambda);  // the wavelength is fixed: looked up once
-    }
+    photon_resume(P, photon_start(P, tbl, i, kNoGlass), kNoGlass, st);
+}
+
+// Glass of the scene's first dielectric material, or kNoGlass.
+__device__ inline uint32_t scene_first_glass(const SceneData &S) {
+    for (uint32_t k = 0; k < S.n_mats; ++k)
+        if (S.mats[k].kind == SPX_MAT_DIELECTRIC) return S.mats[k].glass;
+    return kNoGlass;
 }
 
 // A deposit at film pixel (px, py) = index idx; sinks keyed by film tile
