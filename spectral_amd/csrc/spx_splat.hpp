@@ -54,6 +54,18 @@ struct LdsSplat {
         bool inserted = false;
         for (int p = 0; p < PROBES; ++p) {
             uint32_t s = (h + (uint32_t)p) & (uint32_t)(SLOTS - 1);
+            if (PROBES <= 2) {
+                // Short probes: one compare-and-swap per probe (it returns the
+                // key when the slot is taken) instead of a read and then a
+                // compare-and-swap, one LDS round trip fewer per new key.
+                const uint32_t old = atomicCAS(keys + s, kEmptyKey, idx);
+                if (old == kEmptyKey || old == idx) {
+                    slot = (int)s;
+                    inserted = old == kEmptyKey;
+                    break;
+                }
+                continue;
+            }
             uint32_t k = keys[s];
             if (k == idx) {
                 slot = (int)s;
