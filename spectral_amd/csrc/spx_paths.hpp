@@ -385,6 +385,14 @@ __device__ inline const float *uniform_ptr(const float *p) {
     return (const float *)(uintptr_t)(((uint64_t)hi << 32) | lo);
 }
 
+// phase_hg (volume.rs:88-92) with x^1.5 = x * sqrt(x) and a hardware
+// reciprocal: within a few ulp.
+__device__ inline float phase_hg_approx(float g, float cos_theta) {
+    float g2 = g * g;
+    float x = fmaxf(1.0f + g2 - 2.0f * g * cos_theta, 1e-6f);
+    return (1.0f - g2) * __builtin_amdgcn_rcpf(12.566371f * (x * __builtin_amdgcn_sqrtf(x)));
+}
+
 // Point / legacy estimator deposits for one segment (volume.rs:130-150,
 // 51-64).  ISO: g == 0, where phase_hg(0, c) = 1 / (4 pi * 1^1.5) for every
 // c, bit for bit.  Without a z-buffer every depth passes (+INF).
@@ -407,11 +415,16 @@ __device__ inline Dep point_sample(const PhotonParams &P, const PhotonState &st,
         dp.v = cmf * contrib;
     } else {
         f3 to = cam_o - p;
-        float d = fmaxf(length(to), 1e-3f);
-        if (zbuf && !(d < zbuf[dp.idx])) return dp;
-        float phase = ISO ? 1.0f / 12.566371f : phase_hg(P.vp.g, dot(st.rd, to / d));
-        float trans = expf(-P.vp.sigma_t * d);
-        float contrib = st.power * P.vp.sigma_s * phase * trans / (d * d) * seg_len / 4.0f;
+        const float d2 = dot(to, to);
+        // the z-buffer test (whether the deposit happens) on the IEEE distance
+        if (zbuf && !(fmaxf(sqrtf(d2), 1e-3f) < zbuf[dp.idx])) return dp;
+        // the weight (how much it deposits) from the hardware square root,
+        // reciprocal and exp2, each within a few ulp
+        const float d = fmaxf(__builtin_amdgcn_sqrtf(d2), 1e-3f);
+        const float inv_d = __builtin_amdgcn_rcpf(d);
+        float phase = ISO ? 1.0f / 12.566371f : phase_hg_approx(P.vp.g, dot(st.rd, to * inv_d));
+        float trans = __expf(-P.vp.sigma_t * d);
+        float contrib = st.power * P.vp.sigma_s * phase * trans * (inv_d * inv_d) * seg_len * 0.25f;
         dp.v = cmf * contrib;
     }
     dp.ok = true;
@@ -567,14 +580,6 @@ __device__ inline bool project_approx(const Proj &P, f3 p, float &s, float &t) {
         return true;
     }
     return false;
-}
-
-// phase_hg (volume.rs:88-92) with x^1.5 = x * sqrt(x) and a hardware
-// reciprocal: within a few ulp.
-__device__ inline float phase_hg_approx(float g, float cos_theta) {
-    float g2 = g * g;
-    float x = fmaxf(1.0f + g2 - 2.0f * g * cos_theta, 1e-6f);
-    return (1.0f - g2) * __builtin_amdgcn_rcpf(12.566371f * (x * __builtin_amdgcn_sqrtf(x)));
 }
 
 // FAST: the step's weight and radius -- the values that scale its splat --
