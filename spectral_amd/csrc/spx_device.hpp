@@ -169,8 +169,9 @@ struct Hit {
     bool front_face;
 };
 
-// Sphere::intersect, geom.rs:35-57
-__device__ inline bool sphere_hit(f3 c, float radius, f3 o, f3 d, float t_min, float t_max, Hit &h) {
+// Sphere::intersect, geom.rs:35-57: the hit distance (the point and normal
+// are formed by scene_hit for the nearest primitive only).
+__device__ inline bool sphere_t(f3 c, float radius, f3 o, f3 d, float t_min, float t_max, float &t_out) {
     f3 oc = o - c;
     float a = dot(d, d);
     float half_b = dot(oc, d);
@@ -183,68 +184,88 @@ __device__ inline bool sphere_hit(f3 c, float radius, f3 o, f3 d, float t_min, f
         t = (-half_b + sq) / a;
         if (t < t_min || t > t_max) return false;
     }
-    f3 p = o + d * t;
-    f3 outward = (p - c) / radius;
-    bool front = dot(outward, d) < 0.0f;
-    h.t = t;
-    h.point = p;
-    h.normal = front ? outward : -outward;
-    h.front_face = front;
+    t_out = t;
     return true;
 }
 
-// ConvexSolid::intersect (slab method), geom.rs:128-175
-__device__ inline bool solid_hit(const spx_plane *pl, uint32_t n, f3 o, f3 d, float t_min, float t_max,
-                                 Hit &h) {
+// ConvexSolid::intersect (slab method), geom.rs:128-175: the hit distance
+// and the index of the plane it lies on (n: none, whose normal is zero, as
+// the reference's initial entering / exiting normal).  Every lane walks all
+// n planes: a ray parallel to a plane it lies outside of (the reference's
+// early `return None`) is flagged and rejected after the loop, and each
+// plane's update is a select on its index, so the loop has no per-lane exit
+// and no per-lane branches; the distances, comparisons and result are the
+// reference's.
+__device__ inline bool solid_t(const spx_plane *pl, uint32_t n, f3 o, f3 d, float t_min, float t_max,
+                               float &t_out, uint32_t &plane_out) {
     float t_enter = -INFINITY, t_exit = INFINITY;
-    f3 en = mk(0.f, 0.f, 0.f), ex = mk(0.f, 0.f, 0.f);
+    uint32_t ie = n, ix = n;
+    bool outside = false;
     for (uint32_t i = 0; i < n; ++i) {
-        f3 pn = ld3(pl[i].normal);
-        float denom = dot(pn, d);
-        float dist = dot(pn, o) + pl[i].d;
-        if (fabsf(denom) < 1e-8f) {
-            if (dist > 0.0f) return false;
-            continue;
-        }
-        float t = -dist / denom;
-        if (denom < 0.0f) {
-            if (t > t_enter) { t_enter = t; en = pn; }
-        } else {
-            if (t < t_exit) { t_exit = t; ex = pn; }
-        }
+        const f3 pn = ld3(pl[i].normal);
+        const float denom = dot(pn, d);
+        const float dist = dot(pn, o) + pl[i].d;
+        const bool parallel = fabsf(denom) < 1e-8f;
+        outside = outside || (parallel && dist > 0.0f);
+        const float t = -dist / denom;
+        const bool enter = !parallel && denom < 0.0f && t > t_enter;
+        const bool leave = !parallel && !(denom < 0.0f) && t < t_exit;
+        t_enter = enter ? t : t_enter;
+        ie = enter ? i : ie;
+        t_exit = leave ? t : t_exit;
+        ix = leave ? i : ix;
     }
-    if (t_enter > t_exit) return false;
+    if (outside || t_enter > t_exit) return false;
     float th;
-    f3 hn;
-    if (t_enter > t_min) { th = t_enter; hn = en; }
-    else if (t_exit > t_min) { th = t_exit; hn = ex; }
-    else return false;
+    uint32_t ih;
+    if (t_enter > t_min) {
+        th = t_enter;
+        ih = ie;
+    } else if (t_exit > t_min) {
+        th = t_exit;
+        ih = ix;
+    } else {
+        return false;
+    }
     if (th > t_max) return false;
-    bool front = dot(hn, d) < 0.0f;
-    h.t = th;
-    h.point = o + d * th;
-    h.normal = front ? hn : -hn;
-    h.front_face = front;
+    t_out = th;
+    plane_out = ih;
     return true;
 }
 
-// Scene::intersect, scene.rs:64-78.  Returns the material index or -1.
+// Scene::intersect, scene.rs:64-78.  Returns the material index or -1.  The
+// nearest primitive's hit point and normal (Sphere / ConvexSolid::intersect's
+// HitRecord, geom.rs:50-56, 165-174) are formed once, after the loop, from its
+// distance.
 __device__ inline int scene_hit(const SceneData &s, f3 o, f3 d, Hit &best) {
-    int mat = -1;
     float closest = INFINITY;
+    int bp = -1;
+    uint32_t bplane = 0;
     for (uint32_t i = 0; i < s.n_prims; ++i) {
         const spx_prim &p = s.prims[i];
-        Hit h;
+        float t;
+        uint32_t plane = 0;
         bool ok;
-        if (p.kind == SPX_PRIM_SPHERE) ok = sphere_hit(ld3(p.center), p.radius, o, d, 1e-3f, closest, h);
-        else ok = solid_hit(s.planes + p.plane_start, p.plane_count, o, d, 1e-3f, closest, h);
+        if (p.kind == SPX_PRIM_SPHERE) ok = sphere_t(ld3(p.center), p.radius, o, d, 1e-3f, closest, t);
+        else ok = solid_t(s.planes + p.plane_start, p.plane_count, o, d, 1e-3f, closest, t, plane);
         if (ok) {
-            closest = h.t;
-            best = h;
-            mat = (int)p.material;
+            closest = t;
+            bp = (int)i;
+            bplane = plane;
         }
     }
-    return mat;
+    if (bp < 0) return -1;
+    const spx_prim &p = s.prims[bp];
+    const f3 point = o + d * closest;
+    f3 outward;
+    if (p.kind == SPX_PRIM_SPHERE) outward = (point - ld3(p.center)) / p.radius;
+    else outward = bplane < p.plane_count ? ld3(s.planes[p.plane_start + bplane].normal) : mk(0.f, 0.f, 0.f);
+    const bool front = dot(outward, d) < 0.0f;
+    best.t = closest;
+    best.point = point;
+    best.normal = front ? outward : -outward;
+    best.front_face = front;
+    return (int)p.material;
 }
 
 // background_radiance, scene.rs:44-50, plus the sun-disk extension.
