@@ -402,12 +402,18 @@ template <bool LEGACY, bool ISO>
 __device__ inline Dep point_sample(const PhotonParams &P, const PhotonState &st, f3 cmf, float seg_len,
                                    uint32_t &rng, f3 cam_o, const float *zbuf) {
     Dep dp;
-    dp.ok = false;
     float dist = next_f32(rng) * seg_len;
     f3 p = st.ro + st.rd * dist;
-    float s, t;
-    if (!project(P.proj, p, s, t)) return dp;
-    pix_xy(s, t, P.w, P.h, dp.px, dp.py);
+    // Camera::project (project(), camera.rs:41-69) with its tests folded into
+    // dp.ok instead of returning early: a sample off the film still forms its
+    // (unused) pixel and weight, so a segment's four samples run without
+    // per-lane branches.  The same operations decide the pixel and the tests.
+    const f3 dir = p - ld3(P.proj.o);
+    const float c = dot(dir, ld3(P.proj.w));
+    const float ss = 0.5f + dot(dir, ld3(P.proj.u)) / (-c * P.proj.vw);
+    const float tt = 0.5f + dot(dir, ld3(P.proj.v)) / (-c * P.proj.vh);
+    bool ok = !(c >= -1e-6f) & (ss >= 0.0f) & (ss < 1.0f) & (tt >= 0.0f) & (tt < 1.0f);
+    pix_xy(ok ? ss : 0.0f, ok ? tt : 0.0f, P.w, P.h, dp.px, dp.py);  // in range either way
     dp.idx = dp.py * P.w + dp.px;
     if (LEGACY) {
         const float inv4pi = 1.0f / 12.566371f;
@@ -417,7 +423,7 @@ __device__ inline Dep point_sample(const PhotonParams &P, const PhotonState &st,
         f3 to = cam_o - p;
         const float d2 = dot(to, to);
         // the z-buffer test (whether the deposit happens) on the IEEE distance
-        if (zbuf && !(fmaxf(sqrtf(d2), 1e-3f) < zbuf[dp.idx])) return dp;
+        if (zbuf) ok = ok & (fmaxf(sqrtf(d2), 1e-3f) < zbuf[dp.idx]);
         // the weight (how much it deposits) from the hardware square root,
         // reciprocal and exp2, each within a few ulp
         const float d = fmaxf(__builtin_amdgcn_sqrtf(d2), 1e-3f);
@@ -427,7 +433,7 @@ __device__ inline Dep point_sample(const PhotonParams &P, const PhotonState &st,
         float contrib = st.power * P.vp.sigma_s * phase * trans * (inv_d * inv_d) * seg_len * 0.25f;
         dp.v = cmf * contrib;
     }
-    dp.ok = true;
+    dp.ok = ok;
     return dp;
 }
 
