@@ -11,6 +11,16 @@ def main() -> None:
     for name, sym in (("cie_1931_2deg.csv", "kCmfCsv"), ("illuminant_d65.csv", "kD65Csv"),
                       ("illuminant_a.csv", "kACsv")):
         text = (data / name).read_text()
+        # the device's table lookup (cmf_at, ill_at) blends the last row with
+        # itself at fraction 0, which returns every value but -0 unchanged
+        for line in text.splitlines():
+            for field in line.split(",")[1:]:
+                try:
+                    v = float(field)
+                except ValueError:
+                    continue
+                if v == 0.0 and str(field).strip().startswith("-"):
+                    raise SystemExit(f"{name}: -0 in {line!r}")
         body = "".join(f'    "{line}\\n"\n' for line in text.splitlines())
         parts.append(f"static const char {sym}[] =\n{body};\n")
     new = "// generated by gen_tables.py from spectral_amd/data -- do not edit\n" + "".join(parts)

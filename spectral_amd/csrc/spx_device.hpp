@@ -130,20 +130,23 @@ __device__ inline TblPos tbl_pos(float nm) {
     int i = (int)floorf(f);
     return TblPos{i, f - (float)i, i + 1 >= kTblN};
 }
-// Table::sample on the three CMF columns.
+// Table::sample on the three CMF columns, without a branch for the last row
+// (730 nm): there the position's fraction is exactly 0 (tbl_pos clamps to
+// 70) and the second row read is the last row itself, so the blend
+// a + (a - a) * 0 is a, bit for bit, for every entry but -0, which the
+// tables do not hold (gen_tables.py checks).
 __device__ inline f3 cmf_at(const float4 *tbl, TblPos p) {
-    float4 a = tbl[p.i];
-    if (p.last) return f3{a.x, a.y, a.z};
-    float4 b = tbl[p.i + 1];
+    const float4 a = tbl[p.i];
+    const float4 b = tbl[p.last ? p.i : p.i + 1];
     return f3{a.x + (b.x - a.x) * p.frac, a.y + (b.y - a.y) * p.frac, a.z + (b.z - a.z) * p.frac};
 }
 __device__ inline f3 cmf_lookup(const float4 *tbl, float nm) { return cmf_at(tbl, tbl_pos(nm)); }
-// Table::sample on the illuminant column (1.0 for equal energy).
+// Table::sample on the illuminant column (1.0 for equal energy, a job-wide
+// branch); the last row as in cmf_at.
 __device__ inline float ill_at(const float4 *tbl, TblPos p, bool equal_energy) {
     if (equal_energy) return 1.0f;
-    float a = tbl[p.i].w;
-    if (p.last) return a;
-    float b = tbl[p.i + 1].w;
+    const float a = tbl[p.i].w;
+    const float b = tbl[p.last ? p.i : p.i + 1].w;
     return a + (b - a) * p.frac;
 }
 __device__ inline float ill_lookup(const float4 *tbl, float nm, bool equal_energy) {
