@@ -173,22 +173,22 @@ struct Hit {
 };
 
 // Sphere::intersect, geom.rs:35-57: the hit distance (the point and normal
-// are formed by scene_hit for the nearest primitive only).
+// are formed by scene_hit for the nearest primitive only).  Both roots are
+// computed and the near one taken when it lies in range, instead of
+// branching (a miss, the far root): the same roots, tests and result.
 __device__ inline bool sphere_t(f3 c, float radius, f3 o, f3 d, float t_min, float t_max, float &t_out) {
     f3 oc = o - c;
     float a = dot(d, d);
     float half_b = dot(oc, d);
     float cc = dot(oc, oc) - radius * radius;
     float disc = half_b * half_b - a * cc;
-    if (disc < 0.0f) return false;
-    float sq = sqrtf(disc);
-    float t = (-half_b - sq) / a;
-    if (t < t_min || t > t_max) {
-        t = (-half_b + sq) / a;
-        if (t < t_min || t > t_max) return false;
-    }
-    t_out = t;
-    return true;
+    const float sq = sqrtf(disc);
+    const float t0 = (-half_b - sq) / a;
+    const float t1 = (-half_b + sq) / a;
+    const bool in0 = !(t0 < t_min || t0 > t_max);
+    const bool in1 = !(t1 < t_min || t1 > t_max);
+    t_out = in0 ? t0 : t1;
+    return !(disc < 0.0f) & (in0 | in1);
 }
 
 // ConvexSolid::intersect (slab method), geom.rs:128-175: the hit distance
