@@ -284,30 +284,30 @@ __device__ inline float background_radiance(const spx_env &e, f3 dir) {
 // ---------------------------------------------------------------------------
 __device__ inline f3 reflect(f3 d, f3 n) { return d - n * (2.0f * dot(d, n)); }
 
+// refract and fresnel_reflectance take their branches as selects (the
+// total-internal-reflection outcomes are computed and discarded), so a
+// wave's lanes stay together; the same operations and results (refract
+// writes `out` either way: a NaN direction where it returns false).
 __device__ inline bool refract(f3 d, f3 n, float n1, float n2, f3 &out) {
-    float eta = n1 / n2;
-    f3 nn = n;
-    float cos_i = -dot(d, n);
-    if (cos_i < 0.0f) {
-        nn = -n;
-        cos_i = -cos_i;
-    }
-    float k = 1.0f - eta * eta * (1.0f - cos_i * cos_i);
-    if (k < 0.0f) return false;
-    float cos_t = sqrtf(k);
+    const float eta = n1 / n2;
+    const float c0 = -dot(d, n);
+    const bool flip = c0 < 0.0f;
+    const f3 nn = flip ? -n : n;
+    const float cos_i = flip ? -c0 : c0;
+    const float k = 1.0f - eta * eta * (1.0f - cos_i * cos_i);
+    const float cos_t = sqrtf(k);
     out = normalize(d * eta + nn * (eta * cos_i - cos_t));
-    return true;
+    return !(k < 0.0f);
 }
 
 __device__ inline float fresnel_reflectance(float cos_i, float n1, float n2) {
     cos_i = cos_i < 0.0f ? 0.0f : (cos_i > 1.0f ? 1.0f : cos_i);
-    float r = n1 / n2;
-    float sin_t2 = r * r * (1.0f - cos_i * cos_i);
-    if (sin_t2 >= 1.0f) return 1.0f;
-    float cos_t = sqrtf(1.0f - sin_t2);
-    float qs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
-    float qp = (n1 * cos_t - n2 * cos_i) / (n1 * cos_t + n2 * cos_i);
-    return 0.5f * (qs * qs + qp * qp);
+    const float r = n1 / n2;
+    const float sin_t2 = r * r * (1.0f - cos_i * cos_i);
+    const float cos_t = sqrtf(1.0f - sin_t2);
+    const float qs = (n1 * cos_i - n2 * cos_t) / (n1 * cos_i + n2 * cos_t);
+    const float qp = (n1 * cos_t - n2 * cos_i) / (n1 * cos_t + n2 * cos_i);
+    return sin_t2 >= 1.0f ? 1.0f : 0.5f * (qs * qs + qp * qp);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,12 +339,14 @@ __device__ inline float scatter(const spx_material &m, f3 wo, const Hit &h, floa
     float n1 = h.front_face ? 1.0f : n_hero;
     float n2 = h.front_face ? n_hero : 1.0f;
     float r = fresnel_reflectance(cos_i, n1, n2);
-    if (next_f32(rng) < r) {
-        dir = reflect(wo, h.normal);
-    } else if (!refract(wo, h.normal, n1, n2, dir)) {
-        dir = reflect(wo, h.normal);
-        tir = true;
-    }
+    // reflect when the draw says so or refraction fails (material.rs:66-78),
+    // as selects: both directions are formed, one is kept
+    const bool reflected = next_f32(rng) < r;
+    const f3 refl = reflect(wo, h.normal);
+    f3 refr;
+    const bool refracted = refract(wo, h.normal, n1, n2, refr);
+    dir = reflected || !refracted ? refl : refr;
+    tir = !reflected && !refracted;
     return 1.0f;
 }
 
