@@ -83,12 +83,13 @@ struct TileSink {
 
 // v = (weight * tent weight) in every channel (splat_transverse's colour
 // times the texel's weight, with colour = the step's scalar weight).
+// Every deposit of a sub-group's splats lies in its box: the box is the
+// union of the steps' [x0, x1] x [y0, y1] (the same expressions as
+// splat_transverse's), the tent rows stay inside those, and the
+// rounded-centre fallback round(c) lies in [floor(c - r), ceil(c + r)] for
+// r >= 1/2, clamped to the film like the box.
 __device__ inline void deposit_xy(TileSink &s, uint32_t px, uint32_t py, uint32_t idx, f3 v) {
     const int x = (int)px - s.bx0, y = (int)py - s.by0;
-    if ((uint32_t)x >= (uint32_t)s.bw || (uint32_t)y >= (uint32_t)s.bh) {
-        film_add(s.film, idx, s.cmf * v.x);
-        return;
-    }
     __hip_atomic_fetch_add(s.tile + y * s.bw + x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
@@ -182,13 +183,26 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                 // film floats (pixel f / 3, channel f % 3), so one atomic
                 // instruction covers 64 consecutive floats of a film row (two
                 // or three cache lines) rather than one channel of 64 pixels.
-                for (int f0 = 0; f0 < 3 * area; f0 += 64) {
+                // The quotients from float reciprocals are exact: (f + 1/2) / 3
+                // lies at least 1/6 from an integer and below 1280 + 22, and
+                // (e + 1/2) / bw at least 1/(2 bw) from one and below
+                // area / bw <= 1280 / bw; the reciprocal and the product round
+                // by less than 2^-22 relative, 1302 * 2^-22 < 1/6 and
+                // 1280 / bw * 2^-22 < 1/(2 bw), so the floors are the integer
+                // quotients.
+                const int n3 = 3 * area;
+                const float rbw = __builtin_amdgcn_rcpf((float)bw);
+                for (int f0 = 0; f0 < n3; f0 += 64) {
                     const int f = f0 + (int)lane;
-                    const int e = f / 3, ch = f - 3 * e;
-                    const float v = f < 3 * area ? tile[e] : 0.0f;
+                    const int e = (int)(((float)f + 0.5f) * (1.0f / 3.0f));
+                    const int ch = f - 3 * e;
+                    const float v = f < n3 ? tile[e] : 0.0f;
                     if (v != 0.0f) {
+                        const int row = (int)(((float)e + 0.5f) * rbw);
+                        const int col = e - (int)__umul24((uint32_t)row, (uint32_t)bw);
+                        const uint32_t pix = __umul24((uint32_t)(by0 + row), (uint32_t)W) + (uint32_t)(bx0 + col);
                         const float cc = ch == 0 ? cmf.x : (ch == 1 ? cmf.y : cmf.z);
-                        unsafeAtomicAdd(film + 3 * (size_t)((by0 + e / bw) * W + bx0 + e % bw) + ch, cc * v);
+                        unsafeAtomicAdd(film + (3u * pix + (uint32_t)ch), cc * v);
                     }
                 }
                 lo += S;
