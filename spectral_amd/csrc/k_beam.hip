@@ -93,6 +93,59 @@ __device__ inline void deposit_xy(TileSink &s, uint32_t px, uint32_t py, uint32_
     __hip_atomic_fetch_add(s.tile + y * s.bw + x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// The steps of a segment that can land on the film: [k_lo, k_hi], empty
+// when k_lo > k_hi.  project() rejects a point past any of five planes
+// through the camera -- behind it (dir.w >= -1e-6), or projecting left of,
+// right of, below or above the film (s < 0 is dir.u < (dir.w) vw / 2, and so
+// on) -- and each is a linear function f of the point, positive on the
+// rejected side, so linear in the step index along the segment.  Where f
+// exceeds 2M at both end steps no step lands; where it exceeds 2M at one
+// end, the steps up to one before where it falls to 2M cannot land.  M,
+// 1e-4 of a bound on every term (the points' l1 distance from the camera
+// times 1 + vw + vh), is far above the rounding of f here, of the steps'
+// points and of project()'s own arithmetic (a few 2^-24 of that bound
+// each), so every step cut is one that project() rejects: the film is the
+// one of walking every step.
+__device__ inline void beam_plane_values(const Proj &J, f3 q, float f[5]) {
+    const f3 dir = q - ld3(J.o);
+    const float a = dot(dir, ld3(J.u)), b = dot(dir, ld3(J.v)), c = dot(dir, ld3(J.w));
+    f[0] = c + 1e-6f;
+    f[1] = 0.5f * c * J.vw - a;
+    f[2] = a + 0.5f * c * J.vw;
+    f[3] = 0.5f * c * J.vh - b;
+    f[4] = b + 0.5f * c * J.vh;
+}
+
+__device__ inline void beam_visible_steps(const Proj &J, f3 ro, f3 rd, float ds, uint32_t m, uint32_t &k_lo,
+                                          uint32_t &k_hi) {
+    const f3 o = ld3(J.o);
+    const float da = 0.5f * ds, db = ((float)(m - 1) + 0.5f) * ds;
+    const float l1 = fabsf(ro.x) + fabsf(ro.y) + fabsf(ro.z) + fabsf(o.x) + fabsf(o.y) + fabsf(o.z) +
+                     fmaxf(fabsf(da), fabsf(db)) * (fabsf(rd.x) + fabsf(rd.y) + fabsf(rd.z));
+    const float m2 = 2e-4f * l1 * (1.0f + fabsf(J.vw) + fabsf(J.vh));
+    float fa[5], fb[5];
+    beam_plane_values(J, ro + rd * da, fa);
+    beam_plane_values(J, ro + rd * db, fb);
+    const float span = (float)(m - 1);
+    int lo = 0, hi = (int)m - 1;
+    for (int i = 0; i < 5; ++i) {
+        const bool ha = fa[i] > m2, hb = fb[i] > m2;
+        if (ha && hb) {
+            lo = (int)m;
+            hi = -1;
+        } else if (ha) {  // the first steps lie past plane i
+            const int kc = (int)floorf((fa[i] - m2) / (fa[i] - fb[i]) * span) - 1;
+            lo = max(lo, kc + 1);
+        } else if (hb) {  // the last steps do
+            const int kc = (int)floorf((fb[i] - m2) / (fb[i] - fa[i]) * span) - 1;
+            hi = min(hi, (int)m - 2 - kc);
+        }
+    }
+    k_lo = (uint32_t)lo;
+    k_hi = hi < lo ? 0u : (uint32_t)hi;
+    if (hi < lo) k_lo = 1u;
+}
+
 __device__ inline int wave_min(int v) {
     for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
     return v;
@@ -129,10 +182,12 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
         const uint32_t m = __float_as_uint(c.y);
         const float ds = a.w / (float)m;
         const f3 cmf = cmf_lookup(tbl, c.x);
-        for (uint32_t k0 = 0; k0 < m; k0 += 64) {
+        uint32_t k_lo, k_hi;
+        beam_visible_steps(P.proj, ro, rd, ds, m, k_lo, k_hi);
+        for (uint32_t k0 = k_lo; k0 <= k_hi; k0 += 64) {
             const uint32_t k = k0 + lane;
             BeamSplat sp;
-            const bool ok = k < m && beam_step_prep<kFastTent>(P, ro, rd, b.w, cmf, ds, k, sp);
+            const bool ok = k <= k_hi && beam_step_prep<kFastTent>(P, ro, rd, b.w, cmf, ds, k, sp);
             // the footprint splat_transverse walks (plus its rounded-centre fallback)
             int x0 = 0x7FFFFFFF, x1 = -1, y0 = 0x7FFFFFFF, y1 = -1;
             if (ok) {
