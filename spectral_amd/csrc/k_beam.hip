@@ -146,13 +146,33 @@ __device__ inline void beam_visible_steps(const Proj &J, f3 ro, f3 rd, float ds,
     if (hi < lo) k_lo = 1u;
 }
 
-__device__ inline int wave_min(int v) {
-    for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
-    return v;
+// The deposit kernel's block is one wave, and one wave's LDS operations
+// execute in the order it issues them, so its tile needs no barrier between
+// zeroing, splatting and flushing -- only the compiler kept from moving
+// memory accesses across these points.  (A block barrier would also make
+// the wave wait for its flush's film atomics.)
+__device__ inline void wave_lds_order() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
 }
-__device__ inline int wave_max(int v) {
-    for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
-    return v;
+
+// Wave-wide min / max for a block of one full wave (every lane active):
+// DPP within each 16-lane row (pairs, quads, rotations by 4 and 8), then
+// the four rows' results read into scalar registers -- no LDS round trip.
+template <int CTRL>
+__device__ inline int dpp_i(int x) {
+    return __builtin_amdgcn_update_dpp(0, x, CTRL, 0xF, 0xF, false);
+}
+template <bool MAX>
+__device__ inline int wave_extreme(int v) {
+    auto op = [](int a, int b) { return MAX ? max(a, b) : min(a, b); };
+    v = op(v, dpp_i<0xB1>(v));   // quad_perm [1, 0, 3, 2]
+    v = op(v, dpp_i<0x4E>(v));   // quad_perm [2, 3, 0, 1]
+    v = op(v, dpp_i<0x124>(v));  // row_ror:4
+    v = op(v, dpp_i<0x128>(v));  // row_ror:8
+    const int r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+    const int r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+    return op(op(r0, r1), op(r2, r3));
 }
 
 // Persistent one-wave blocks: a wave claims the next queued segment and
@@ -174,7 +194,7 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
     while (true) {
         uint32_t j = 0;
         if (lane == 0) j = atomicAdd(cursor, 1u);
-        j = __builtin_amdgcn_readfirstlane(__shfl(j, 0));
+        j = __builtin_amdgcn_readlane(j, 0);
         if (j >= total) break;
         const float4 *q = segs + (size_t)j * kBeamSegFloat4;
         const float4 a = q[0], b = q[1], c = q[2];
@@ -207,10 +227,10 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
             uint32_t lo = 0, S = 64;
             while (lo < 64) {
                 const bool in = lane >= lo && lane < lo + S;
-                const int bx0 = __builtin_amdgcn_readfirstlane(wave_min(in ? x0 : 0x7FFFFFFF));
-                const int bx1 = __builtin_amdgcn_readfirstlane(wave_max(in ? x1 : -1));
-                const int by0 = __builtin_amdgcn_readfirstlane(wave_min(in ? y0 : 0x7FFFFFFF));
-                const int by1 = __builtin_amdgcn_readfirstlane(wave_max(in ? y1 : -1));
+                const int bx0 = wave_extreme<false>(in ? x0 : 0x7FFFFFFF);
+                const int bx1 = wave_extreme<true>(in ? x1 : -1);
+                const int by0 = wave_extreme<false>(in ? y0 : 0x7FFFFFFF);
+                const int by1 = wave_extreme<true>(in ? y1 : -1);
                 if (bx1 < 0) {  // no step of the sub-group is on the film
                     lo += S;
                     continue;
@@ -227,13 +247,13 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                 }
                 const int area = bw * bh;
                 for (int e = (int)lane; e < area; e += 64) tile[e] = 0.0f;
-                __syncthreads();
+                wave_lds_order();
                 if (in && ok) {
                     TileSink ts{tile, film, cmf, bx0, by0, bw, bh};
                     splat_transverse<TileSink, kFastTent>(P, sp.s, sp.t, mk(sp.weight, sp.weight, sp.weight),
                                                           sp.radius, ts);
                 }
-                __syncthreads();
+                wave_lds_order();
                 // Flush: lane l takes float f = f0 + l of the box's 3 * area
                 // film floats (pixel f / 3, channel f % 3), so one atomic
                 // instruction covers 64 consecutive floats of a film row (two
@@ -261,7 +281,7 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                     }
                 }
                 lo += S;
-                __syncthreads();
+                wave_lds_order();
             }
         }
     }
