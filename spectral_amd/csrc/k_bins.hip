@@ -78,12 +78,12 @@ __device__ inline uint32_t slot_add(uint32_t *cnt, uint32_t slot, bool active) {
     uint64_t m = __ballot(active);
     if (!m) return 0;
     int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t ls = __shfl(slot, leader);
+    uint32_t ls = __builtin_amdgcn_readlane(slot, leader);
     if (__ballot(active && slot == ls) == m) {
         uint32_t lane = __lane_id();
         uint32_t before = 0;
         if ((int)lane == leader) before = atomicAdd(cnt + ls, (uint32_t)__popcll(m));
-        before = __shfl(before, leader);
+        before = __builtin_amdgcn_readlane(before, leader);
         return before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     }
     return active ? atomicAdd(cnt + slot, 1u) : 0u;

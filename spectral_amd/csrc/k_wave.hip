@@ -69,7 +69,7 @@ __device__ inline uint32_t region_enqueue(bool alive, uint32_t *lds_count) {
     if (total) {
         int leader = __ffsll((unsigned long long)mask) - 1;
         if ((int)lane == leader) base = atomicAdd(lds_count, total);
-        base = __shfl(base, leader);
+        base = __builtin_amdgcn_readlane(base, leader);
     }
     return base + prefix;
 }

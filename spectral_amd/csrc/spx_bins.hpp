@@ -213,7 +213,7 @@ struct BinSink {
             wstate[2 * w] = c;
             wstate[2 * w + 1] = f;
         }
-        base = __shfl(base, leader);
+        base = __builtin_amdgcn_readlane(base, leader);
         if (base == kChunkFull) {
             film_add(film, idx, v);
             return;
