@@ -423,10 +423,17 @@ hipError_t enqueue(spx_ctx *ctx, const Schedule &S, char *d, const RoundBufs &B,
         ++k;
     }
     // The concurrent camera round's tail: once the photon rounds are traced,
-    // a full grid joins its blocks in draining the camera queue (blocks that
+    // more blocks join its grid in draining the camera queue (blocks that
     // find it empty exit at once), so the step does not wait on a camera
-    // round left with its small grid alone.
-    if (e == hipSuccess && S.cam_blocks) e = trace(0, B.bst[0]);
+    // round left with its small grid alone.  Two blocks per compute unit, not
+    // the full grid: the last rounds' bin passes still run on the second
+    // stream, and persistent camera blocks on every slot starved them (one
+    // scatter pass 24 ms instead of 5).  The 512-job mixed step on 256 CUs,
+    // against the full grid of 768 blocks: 256, 512 and 1024 blocks gave
+    // 244.5, 234.5 and 242.2 ms against 241.1 on one box; 384, 512 and 640
+    // gave 223.7, 221.3 and 224.5 against 227.3 on another.
+    const uint32_t cus = (uint32_t)(ctx->num_cus > 0 ? ctx->num_cus : 256);
+    if (e == hipSuccess && S.cam_blocks) e = trace_on(0, B.bst[0], s, std::min(S.blocks, 2u * cus));
     if (e == hipSuccess && S.overlap && k) e = hipStreamWaitEvent(s, B.ev[2 * (k - 1) + 1], 0);  // join
     if (e == hipSuccess && S.cam_blocks) e = hipStreamWaitEvent(s, B.cev[1], 0);
     if (e == hipSuccess && !S.reps.empty())
