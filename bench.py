@@ -469,8 +469,11 @@ def kernel_roofline(name, alg_bytes_step, ms_step, launches_step, bound="valu"):
     if lanes is not None:
         out["lanes_active"] = lanes
     if busy is not None and lanes is not None:
-        # the share of the SIMDs' lane slots doing useful VALU work
-        out["valu_lane_frac"] = round(busy * lanes / 64.0, 3)
+        # the share of the SIMDs' lane slots doing useful VALU work; the
+        # counters' issue ratio is taken as at most 1 (SQ_ACTIVE_INST_VALU also
+        # counts the extra cycles of multi-cycle instructions, so a saturated
+        # kernel can read above 1; valu_busy keeps the raw ratio)
+        out["valu_lane_frac"] = round(min(busy, 1.0) * lanes / 64.0, 3)
     if atomics is not None and sec > 0:
         out["global_atomics_per_launch"] = atomics
         out["atomics_gops"] = round(atomics / sec / 1e9, 3)

@@ -25,12 +25,13 @@ from .core import Context, DeviceScene, Film, _check, opts
 
 
 # spx_autotune's pick on the 512-job mixed stream (bench.py --autotune, one
-# coordinate sweep; repeated runs pick 768 or 1024 persistent blocks and
-# 4096- or 8192-photon items within 2 % of each other): deposit mode 3,
-# three record buffers cycling over overlapped rounds (256 ms per step
-# against 267 ms with two), a 128-block concurrent camera round.
+# coordinate sweep): deposit mode 3, three record buffers cycling over
+# overlapped rounds, a 128-block concurrent camera round, four persistent
+# blocks per compute unit with 32768-path and 8192-photon items (217.2 ms
+# per step and a 60.3 ms camera half, against 219.6 and 67.4 ms with the
+# previous pick of three blocks, 16384 and 4096).
 TUNED_MIXED512 = {"max_replicas": 16, "replica_mib": 64, "splat_mode": 3, "chunk_log2": 22, "bin_log2": 30,
-                  "tile_wlog2": 5, "batch_blocks": 768, "chunk_paths": 16384, "chunk_photons": 4096,
+                  "tile_wlog2": 5, "batch_blocks": 1024, "chunk_paths": 32768, "chunk_photons": 8192,
                   "overlap_bins": 2, "camera_blocks": 128, "photon_threads": 256}
 
 
