@@ -220,6 +220,7 @@ __global__ __launch_bounds__(64) void k_beam_deposit(PhotonParams P, const float
                 y1 = (int)ceilf(cy + r);
                 y1 = y1 < 0 ? 0 : (y1 > H - 1 ? H - 1 : y1);
             }
+            if (__ballot(ok) == 0) continue;  // no step of the group deposits
             // The group's steps in sub-groups: all 64 when their footprint box
             // fits the tile, else halves, quarters, ... (a single step's box,
             // at most 50 x 50 texels for the 24-pixel radius cap, fits any
