@@ -28,7 +28,7 @@ class PlaneImage:
 
 
 def prism_on_plane(ctx: S.Context, width: int = 800, height: int = 600, n_caustic: int = 40_000_000,
-                   n_volume: int = 35_000_000, floor_side: int = 512, engine: int = A.ENGINE_PLAIN,
+                   n_volume: int = 35_000_000, floor_side: int = 512, engine: int = A.ENGINE_WAVEFRONT,
                    volume_engine: int = A.ENGINE_WAVEFRONT, stream=None) -> PlaneImage:
     sc = scenes.wedge_scene().upload(ctx)       # prism_on_plane.rs:91-93
     scr = scenes.plane_floor_screen(floor_side, floor_side)
